@@ -1,0 +1,230 @@
+// Internal state of the NeuronGPU engine (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "neurongpu.h"
+#include "ngpu_kernels.h"
+
+namespace ngpu {
+
+#define NGPU_HIP(call)                                                      \
+  do {                                                                      \
+    hipError_t err_ = (call);                                               \
+    if (err_ != hipSuccess)                                                 \
+      throw ngpu_exception(std::string("HIP error: ") +                     \
+                           hipGetErrorString(err_) + " in " #call);          \
+  } while (0)
+
+enum ModelId {
+  kAeifCondBeta = 0,
+  kAeifCondAlpha,
+  kIafPscExp,
+  kSpikeGenerator,
+  kPoissonGenerator,
+  kSpikeDetector,
+  kParrotNeuron,
+  kNumModels
+};
+
+struct ModelDesc {
+  const char *name;
+  std::vector<std::string> scal_var, port_var, scal_param, port_param,
+      group_param, array_param;
+  std::vector<float> scal_param_default, port_param_default,
+      group_param_default, scal_var_default, port_var_default;
+  int fixed_ports;  // -1: chosen at Create
+};
+
+const ModelDesc &Model(ModelId id);
+int FindModel(const std::string &name);
+
+// device buffer owned by the engine
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) NGPU_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *h, size_t count, size_t off = 0) {
+    if (count) NGPU_HIP(hipMemcpy(p + off, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void download(T *h, size_t count, size_t off = 0) const {
+    if (count) NGPU_HIP(hipMemcpy(h, p + off, count * sizeof(T), hipMemcpyDeviceToHost));
+  }
+  ~DBuf() { release(); }
+};
+
+struct Group {
+  ModelId model;
+  int i0 = 0, n = 0, n_port = 0;
+  int n_var = 0, n_param = 0;
+  std::vector<float> var;          // host SoA [n_var][n]
+  std::vector<float> param;        // host SoA [n_param][n]
+  std::vector<float> group_param;
+  // spike generator
+  std::vector<std::vector<float> > sg_times, sg_heights;
+  // recording of spikes
+  bool spike_count_on = false;
+  int max_rec_times = 0;
+  std::vector<int> h_spike_count;
+  std::vector<int> h_n_rec_times;
+  std::vector<float> h_rec_times;
+  // device
+  DBuf<float> d_var, d_param;
+  DBuf<double> d_x;
+  DBuf<float> d_h;
+  std::vector<DBuf<float> *> d_derived;  // per-neuron derived parameters
+  DBuf<int> d_sg_n, d_sg_first, d_sg_tidx, d_sg_next;
+  DBuf<float> d_sg_h;
+  DBuf<int> d_spike_count, d_n_rec_times;
+  DBuf<float> d_rec_times;
+  // synchronisation between host mirror and device
+  bool host_newer = true;   // host arrays changed since last upload
+  bool dev_newer = false;   // device advanced since last download
+  bool kparam_dirty = true;
+  // kernel parameters (rebuilt when dirty)
+  AeifParams aeif;
+  IafArgs iaf;
+  PField hold;
+  float rk5_h_min = 0.f, rk5_h0 = 0.f, rk5_eps = 1e-6f;
+  double rk5_h_min_acc = 0.0;
+
+  int VarIdx(const std::string &name, int port) const;
+  int ParamIdx(const std::string &name, int port) const;
+  ~Group();
+};
+
+struct SynGroupDesc {
+  int type;  // 1 test_syn_model, 2 stdp
+  std::vector<std::string> names;
+  std::vector<float> param;
+};
+
+struct Record {
+  std::string file_name;
+  std::vector<std::string> var_name;
+  std::vector<int> node, port;
+  std::vector<std::vector<float> > data;
+  FILE *fp = nullptr;
+};
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+
+  // kernel parameters (src/neurongpu.cu:59-85)
+  float time_resolution = 0.1f;
+  float sim_time = 1000.0f;
+  int verbosity = 4;
+  int max_spike_buffer_size = 20;
+  float max_spike_num_fact = 1.0f;
+  float max_spike_per_host_fact = 1.0f;
+  unsigned long long seed = 54321ULL;
+  int remote_spike_height_flag = 0;
+  unsigned long long rng_calls = 0;  // host random stream counter
+
+  bool calibrated = false;
+  bool first_simulation = true;
+  bool in_simulation = false;
+  double t_min = 0.0, neural_time = 0.0, neur_t0 = 0.0;
+  long long it = 0, Nt = 0, gs = 0;
+  long long T_base = 0;
+
+  std::vector<Group *> groups;
+  std::vector<int> node_group;
+  int n_node = 0;
+
+  // connections, coordinate form, kept until calibration
+  std::vector<uint32_t> c_src, c_tp;  // tp = target | port << 26
+  std::vector<float> c_w;
+  std::vector<uint16_t> c_d;          // delay index d_int
+  std::vector<uint8_t> c_sg;
+  bool any_plastic = false;
+
+  // canonical CSR on the host (built at calibration, or on demand)
+  bool csr_valid = false;
+  int D = 0;                          // delay slots
+  std::vector<unsigned long long> h_base;  // [n_node+1]
+  std::vector<uint32_t> h_seg;        // [n_node][D+1]
+  std::vector<unsigned long long> h_rec;
+  std::vector<uint8_t> h_sg;
+  bool h_rec_stale = false;           // device weights changed (plasticity)
+
+  std::vector<SynGroupDesc> syn_groups;
+
+  // device state
+  hipStream_t stream = nullptr;
+  int max_port = 1;
+  int R = 0;
+  long long ring_cap = 0;
+  DBuf<float> d_acc;
+  DBuf<uint32_t> d_ring_src, d_ring_cnt, d_ring_n;
+  DBuf<float> d_ring_h;
+  DBuf<long long> d_last_spike, d_last_rev;
+  DBuf<float> d_last_height;
+  DBuf<int> d_den_idx;
+  int max_den = 0;
+  DBuf<unsigned long long> d_base, d_rec;
+  DBuf<uint32_t> d_seg;
+  DBuf<uint8_t> d_sg;
+  DBuf<uint16_t> d_spike_time;
+  DBuf<unsigned long long> d_rev_ptr;
+  DBuf<uint32_t> d_rev_idx;
+  DBuf<float> d_stdp_param;
+  DBuf<int> d_syn_type;
+  // Poisson direct connections (CSR by target)
+  long long n_dir = 0;
+  DBuf<uint32_t> d_dir_ptr, d_dir_canon, d_dir_gen, d_gen_table, d_tables;
+  DBuf<float> d_dir_w, d_dir_delay, d_gen_origin, d_gen_start, d_gen_stop;
+  DBuf<uint8_t> d_dir_port;
+  bool pois_dirty = true;
+  // statistics
+  DBuf<unsigned long long> d_cnt_rk5, d_cnt_events;
+  unsigned long long n_spikes_total = 0;
+  double step_seconds = 0.0;
+  // pending end-of-step input (Poisson draws of the last step)
+  long long pending_T = -1;
+  double pending_t = 0.0;
+  // records
+  std::vector<Record> records;
+  int rec_cols = 0;
+  DBuf<float *> d_rec_ptr;
+  DBuf<int> d_rec_node;
+  DBuf<float> d_rec_buf;
+  long long rec_rows_cap = 0, rec_rows = 0;
+  std::vector<double> rec_times;
+
+  // helpers
+  int GroupOf(int i_node, int n_node_req) const;
+  int GroupOfList(const int *nodes, int n) const;
+  Group &G(int i) { return *groups[i]; }
+  void CheckUncalibrated(const char *msg) const;
+  void SyncHost(Group &g);
+  void UploadGroup(Group &g);
+  void BuildKernelParams(Group &g);
+  void BuildCSR();
+  void BuildPoisson();
+  void AllocRecordBuffer(long long rows);
+  void FlushRecords();
+  void WriteHostRecordRow(double t);
+  void LaunchGroup(Group &g, bool do_step, long long T, double t,
+                   long long it_local);
+  void EnsureHostCSR();
+  FoldIn MakeFold(long long pois_T, double pois_t);
+  SpikeOut MakeSpikeOut(Group &g, long long T, double t);
+};
+
+}  // namespace ngpu

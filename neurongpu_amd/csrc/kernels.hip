@@ -1,0 +1,757 @@
+// Per-timestep kernels for gfx950 (MI355X).
+//
+// Step sequence launched by the engine (one HIP stream, no host syncs):
+//   node updates   one launch per node group.  Each first folds the port
+//                  input of the previous step (delivery accumulators plus
+//                  the Poisson draws of that step) into its state, then
+//                  advances the state by dt and appends its spikes to the
+//                  spike ring slot of this step (wavefront ballot + one
+//                  atomic per wavefront).
+//   record         gathers recorded variables into a device row buffer
+//   delivery       flattened CSR-by-source gather over all (spike, delay)
+//                  pairs due this step, float atomics into acc[port][node]
+//   post STDP      (plastic networks only) reverse-index sweep
+//
+// Reference behaviour followed (file:line in the NeuronGPU sources):
+//   fold            src/get_spike.cu:116-140, src/poiss_gen.cu:43-74
+//   RK5             src/rk5.h:57-195, src/rk5.cu:23-60
+//   AdEx RHS/reset  src/aeif_cond_beta_kernel.h:153-214,
+//                   src/aeif_cond_alpha_kernel.h:149-211
+//   iaf_psc_exp     src/iaf_psc_exp.cu:94-119
+//   PushSpike       src/spike_buffer.cu:143-228
+//   delivery        src/spike_buffer.cu:233-302, src/nested_loop.cu:96-167,
+//                   src/get_spike.cu:48-85
+//   post STDP       src/rev_spike.cu:41-75, src/stdp.cu:24-51
+#include <hip/hip_runtime.h>
+
+#include "ngpu_device.h"
+#include "ngpu_kernels.h"
+
+namespace ngpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wavefront-aggregated append of this lane's spikes to the ring slot of the
+// step, plus the per-node bookkeeping of PushSpike.
+__device__ __forceinline__ void emit_spikes(const SpikeOut &o, bool active,
+                                            int i, int node, int nsp,
+                                            float height) {
+  bool spk = active && nsp > 0;
+  unsigned long long mask = __ballot(spk);
+  if (mask == 0ull) return;
+  uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(o.ring_n, (uint32_t)__popcll(mask));
+  base = __shfl(base, (int)leader);
+  if (!spk) return;
+  uint32_t pos = base + lanes_below(mask);
+  o.ring_src[pos] = (uint32_t)node;
+  o.ring_h[pos] = height;
+  o.ring_cnt[pos] = (uint32_t)nsp;
+  o.last_spike[node] = o.T;
+  o.last_height[node] = height / (float)nsp;
+  if (o.den_idx[node] == 0) o.last_rev[node] = o.T;
+  if (o.spike_count) o.spike_count[i] += nsp;
+  if (o.rec_times) {
+    int nr = o.n_rec_times[i];
+    for (int s = 0; s < nsp; s++) {
+      if (nr >= o.max_rec_times - 1) break;
+      o.rec_times[(long long)i * o.max_rec_times + nr] = (float)o.t;
+      nr++;
+    }
+    o.n_rec_times[i] = nr;
+  }
+}
+
+// Port input of the previous step: delivery accumulator of every port plus
+// the Poisson draws of the node's direct connections (canonical order).
+template <int P>
+__device__ __forceinline__ void gather_input(const FoldIn &f, int node,
+                                             float (&a)[P]) {
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    float *ap = f.acc + (long long)p * f.n_total + node;
+    a[p] = *ap;
+    *ap = 0.0f;
+  }
+  if (f.pois_T < 0 || !f.dir.active) return;
+  const DirectIn &d = f.dir;
+  uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
+  uint32_t Tlo = (uint32_t)f.pois_T, Thi = (uint32_t)((uint64_t)f.pois_T >> 32);
+  for (uint32_t j = j0; j < j1; j++) {
+    uint32_t g = d.gen[j];
+    double t_rel = f.pois_t - (double)d.gen_origin[g] - (double)d.delay[j];
+    if (!(t_rel >= (double)d.gen_start[g] && t_rel <= (double)d.gen_stop[g]))
+      continue;
+    uint32_t ti = d.gen_table[g];
+    if (ti == 0xFFFFFFFFu) continue;
+    const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
+    uint32_t u = dev_philox_w0(d.canon[j], Tlo, Thi, kPoissonTag, d.key0,
+                               d.key1);
+    int k = 0;
+    while (u > tab[k]) k++;
+    if (k > 0) {
+      float add = d.weight[j] * (float)k;
+      int port = d.port[j];
+#pragma unroll
+      for (int p = 0; p < P; p++)
+        if (port == p) a[p] = a[p] + add;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// AdEx conductance neurons (aeif_cond_beta / aeif_cond_alpha), RK5.
+// ---------------------------------------------------------------------
+template <int P>
+struct AeifLocal {
+  float V_th, Delta_T, inv_Delta_T, g_L, E_L, inv_C_m, a, b, inv_tau_w, I_e,
+      V_peak, V_reset, t_ref;
+  float E_rev[P], inv_tau_rise[P], inv_tau_decay[P], g0[P];
+};
+
+template <int P>
+__device__ __forceinline__ void aeif_rhs(const AeifLocal<P> &p, float refr,
+                                         const float (&y)[2 + 2 * P],
+                                         float (&dy)[2 + 2 * P]) {
+  bool rf = refr > 0.0f;
+  float V = rf ? p.V_reset : ((y[0] < p.V_peak) ? y[0] : p.V_peak);
+  float I_syn = 0.0f;
+#pragma unroll
+  for (int i = 0; i < P; i++)
+    I_syn = __builtin_fmaf(y[2 + 2 * i], p.E_rev[i] - V, I_syn);
+  float V_spike = p.Delta_T * dev_expf((V - p.V_th) * p.inv_Delta_T);
+  float t = (V - p.E_L) - V_spike;
+  t = (-p.g_L) * t;
+  t = t + I_syn;
+  t = t - y[1];
+  t = t + p.I_e;
+  dy[0] = rf ? 0.0f : t * p.inv_C_m;
+  float u = p.a * (V - p.E_L);
+  u = u - y[1];
+  dy[1] = u * p.inv_tau_w;
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    float g = y[2 + 2 * i], g1 = y[3 + 2 * i];
+    dy[3 + 2 * i] = (-g1) * p.inv_tau_rise[i];
+    dy[2 + 2 * i] = __builtin_fmaf(-g, p.inv_tau_decay[i], g1);
+  }
+}
+
+// One adaptive Cash-Karp step (src/rk5.h:57-147).  Returns false when the
+// attempt cap is reached.
+template <int P>
+__device__ __forceinline__ bool aeif_rk5_step(const AeifLocal<P> &p, float refr,
+                                              double &x, float (&y)[2 + 2 * P],
+                                              float &h, float h_min,
+                                              double h_min_acc, float h_max,
+                                              float eps, int &n_att,
+                                              int &n_acc, int &n_rej) {
+  constexpr int N = 2 + 2 * P;
+  float k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], yn[N], inv_scal[N];
+  float err = 0.0f;
+  aeif_rhs<P>(p, refr, y, k1);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float s = fabsf(y[i]) + fabsf(k1[i] * h);
+    s = s + CK::scal_min;
+    inv_scal[i] = 1.0f / s;
+  }
+  for (;;) {
+    float hh = h;
+    if (hh > h_max) hh = h_max;
+    if (hh < h_min) hh = h_min;
+    h = hh;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      float t = CK::a21 * k1[i];
+      yn[i] = __builtin_fmaf(hh, t, y[i]);
+    }
+    aeif_rhs<P>(p, refr, yn, k2);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      float t = CK::a31 * k1[i];
+      t = __builtin_fmaf(CK::a32, k2[i], t);
+      yn[i] = __builtin_fmaf(hh, t, y[i]);
+    }
+    aeif_rhs<P>(p, refr, yn, k3);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      float t = CK::a41 * k1[i];
+      t = __builtin_fmaf(CK::a42, k2[i], t);
+      t = __builtin_fmaf(CK::a43, k3[i], t);
+      yn[i] = __builtin_fmaf(hh, t, y[i]);
+    }
+    aeif_rhs<P>(p, refr, yn, k4);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      float t = CK::a51 * k1[i];
+      t = __builtin_fmaf(CK::a52, k2[i], t);
+      t = __builtin_fmaf(CK::a53, k3[i], t);
+      t = __builtin_fmaf(CK::a54, k4[i], t);
+      yn[i] = __builtin_fmaf(hh, t, y[i]);
+    }
+    aeif_rhs<P>(p, refr, yn, k5);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      float t = CK::a61 * k1[i];
+      t = __builtin_fmaf(CK::a62, k2[i], t);
+      t = __builtin_fmaf(CK::a63, k3[i], t);
+      t = __builtin_fmaf(CK::a64, k4[i], t);
+      t = __builtin_fmaf(CK::a65, k5[i], t);
+      yn[i] = __builtin_fmaf(hh, t, y[i]);
+    }
+    aeif_rhs<P>(p, refr, yn, k6);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      float t = CK::a71 * k1[i];
+      t = __builtin_fmaf(CK::a73, k3[i], t);
+      t = __builtin_fmaf(CK::a74, k4[i], t);
+      t = __builtin_fmaf(CK::a76, k6[i], t);
+      yn[i] = __builtin_fmaf(hh, t, y[i]);
+    }
+    err = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      float t = CK::e1 * k1[i];
+      t = __builtin_fmaf(CK::e3, k3[i], t);
+      t = __builtin_fmaf(CK::e4, k4[i], t);
+      t = __builtin_fmaf(CK::e5, k5[i], t);
+      t = __builtin_fmaf(CK::e6, k6[i], t);
+      float v = fabsf((hh * t) * inv_scal[i]);
+      err = (err > v) ? err : v;
+    }
+    err = err / eps;
+    n_att++;
+    if (err <= 1.0f || (double)hh <= h_min_acc) break;
+    n_rej++;
+    if (n_att > kMaxAttemptsPerStep) return false;
+    float h_new = (hh * CK::coeff) * dev_pow_dec(err);
+    float h_lo = 0.1f * hh;
+    h = (h_new > h_lo) ? h_new : h_lo;
+  }
+  n_acc++;
+  x += (double)h;
+  if (err > CK::err_min) {
+    h = (h * CK::coeff) * dev_pow_inc(err);
+  } else {
+    h = 5.0f * h;
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) y[i] = yn[i];
+  return true;
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  bool active = i < A.n;
+  int ii = active ? i : 0;
+  int node = A.i0 + ii;
+  int nsp = 0;
+  int n_acc = 0, n_rej = 0;
+  if (active) {
+    constexpr int N = 2 + 2 * P;
+    AeifLocal<P> p;
+    p.V_th = pf(A.p.V_th, i);
+    p.Delta_T = pf(A.p.Delta_T, i);
+    p.inv_Delta_T = pf(A.p.inv_Delta_T, i);
+    p.g_L = pf(A.p.g_L, i);
+    p.E_L = pf(A.p.E_L, i);
+    p.inv_C_m = pf(A.p.inv_C_m, i);
+    p.a = pf(A.p.a, i);
+    p.b = pf(A.p.b, i);
+    p.inv_tau_w = pf(A.p.inv_tau_w, i);
+    p.I_e = pf(A.p.I_e, i);
+    p.V_peak = pf(A.p.V_peak, i);
+    p.V_reset = pf(A.p.V_reset, i);
+    p.t_ref = pf(A.p.t_ref, i);
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      p.E_rev[q] = pf(A.p.E_rev[q], i);
+      p.inv_tau_rise[q] = pf(A.p.inv_tau_rise[q], i);
+      p.inv_tau_decay[q] = pf(A.p.inv_tau_decay[q], i);
+      p.g0[q] = pf(A.p.g0[q], i);
+    }
+    float y[N];
+    y[0] = A.V[i];
+    y[1] = A.w[i];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      y[2 + 2 * q] = A.g[q][i];
+      y[3 + 2 * q] = A.g1[q][i];
+    }
+    // fold the previous step's port input into g1
+    float in[P];
+    gather_input<P>(A.fold, node, in);
+#pragma unroll
+    for (int q = 0; q < P; q++)
+      y[3 + 2 * q] = __builtin_fmaf(in[q], p.g0[q], y[3 + 2 * q]);
+    if (A.do_step) {
+      double x = A.x[i];
+      float h = A.h[i];
+      float refr = A.refr[i];
+      double x1 = A.x1;
+      int n_att = 0;
+      bool end = false;
+      while (!end) {
+        float hmax = (float)(x1 - x);
+        if (!aeif_rk5_step<P>(p, refr, x, y, h, A.h_min, A.h_min_acc, hmax,
+                              A.eps, n_att, n_acc, n_rej))
+          x = x1;
+        end = (x >= x1 - (double)A.h_min);
+        // ExternalUpdate
+        if (y[0] < -1.0e3f) {
+          y[0] = p.V_reset;
+          y[1] = 0.0f;
+          continue;
+        }
+        if (y[1] < -1.0e6f || y[1] > 1.0e6f) {
+          y[0] = p.V_reset;
+          y[1] = 0.0f;
+          continue;
+        }
+        if (refr > 0.0f) {
+          y[0] = p.V_reset;
+          if (end) refr -= 1.0f;
+        } else if (y[0] >= p.V_peak) {
+          nsp++;
+          y[0] = p.V_reset;
+          y[1] = y[1] + p.b;
+          float r = roundf(p.t_ref / A.dt);
+          int ri = (int)r;
+          refr = (float)(ri < 0 ? 0 : ri);
+        }
+      }
+      A.x[i] = x;
+      A.h[i] = h;
+      A.refr[i] = refr;
+    }
+    A.V[i] = y[0];
+    A.w[i] = y[1];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      A.g[q][i] = y[2 + 2 * q];
+      A.g1[q][i] = y[3 + 2 * q];
+    }
+  }
+  if (!A.do_step) return;
+  emit_spikes(A.out, active, ii, node, nsp, (float)nsp);
+  // statistics: accepted / rejected RK5 attempts
+  int sa = n_acc, sr = n_rej;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sa += __shfl_xor(sa, off);
+    sr += __shfl_xor(sr, off);
+  }
+  if (lane_id() == 0 && A.cnt && (sa | sr)) {
+    unsigned long long *c = A.cnt + 2 * (blockIdx.x & (kCounterSlots - 1));
+    atomicAdd(c, (unsigned long long)sa);
+    atomicAdd(c + 1, (unsigned long long)sr);
+  }
+}
+
+// ---------------------------------------------------------------------
+// iaf_psc_exp, exact integration (src/iaf_psc_exp.cu:94-119)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void iaf_kernel(IafArgs A) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  bool active = i < A.n;
+  int ii = active ? i : 0;
+  int node = A.i0 + ii;
+  int nsp = 0;
+  if (active) {
+    float in[2];
+    gather_input<2>(A.fold, node, in);
+    float I_ex = A.I_ex[i] + in[0];
+    float I_in = A.I_in[i] + in[1];
+    if (A.do_step) {
+      float V = A.V[i];
+      float refr = A.refr[i];
+      if (refr > 0.0f) {
+        refr -= 1.0f;
+      } else {
+        float v = V * pf(A.P22, i);
+        v = __builtin_fmaf(I_ex, pf(A.P21ex, i), v);
+        v = __builtin_fmaf(I_in, pf(A.P21in, i), v);
+        v = __builtin_fmaf(pf(A.I_e, i), pf(A.P20, i), v);
+        V = v;
+      }
+      I_ex = I_ex * pf(A.P11ex, i);
+      I_in = I_in * pf(A.P11in, i);
+      if (V >= pf(A.Theta, i)) {
+        nsp = 1;
+        V = pf(A.V_reset, i);
+        refr = (float)(int)roundf(pf(A.t_ref, i) / A.dt);
+      }
+      A.V[i] = V;
+      A.refr[i] = refr;
+    }
+    A.I_ex[i] = I_ex;
+    A.I_in[i] = I_in;
+  }
+  if (!A.do_step) return;
+  emit_spikes(A.out, active, ii, node, nsp, 1.0f);
+}
+
+// ---------------------------------------------------------------------
+// spike_detector / parrot_neuron
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void relay_kernel(RelayArgs A) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  bool active = i < A.n;
+  int ii = active ? i : 0;
+  int node = A.i0 + ii;
+  int nsp = 0;
+  float height = 0.0f;
+  if (active) {
+    float in[2];
+    if (A.parrot) {
+      gather_input<2>(A.fold, node, in);
+    } else {
+      float in1[1];
+      gather_input<1>(A.fold, node, in1);
+      in[0] = in1[0];
+    }
+    float input = A.input[i] + in[0];
+    if (A.do_step) {
+      float sh = input;
+      float hold = pf(A.hold, i);
+      if (A.parrot) {
+        A.out_var[i] = sh;
+        if (sh != 0.0f) {
+          if (hold == 0.0f) sh = 1.0f;
+          input = 0.0f;
+          nsp = 1;
+          height = sh;
+        }
+      } else {
+        if (sh != 0.0f) {
+          if (hold == 0.0f) sh = 1.0f;
+          input = 0.0f;
+        }
+        A.out_var[i] = sh;
+      }
+    }
+    A.input[i] = input;
+  }
+  if (!A.do_step || !A.parrot) return;
+  emit_spikes(A.out, active, ii, node, nsp, height);
+}
+
+// ---------------------------------------------------------------------
+// spike_generator (src/spike_generator.cu:37-55)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  bool active = i < A.n;
+  int ii = active ? i : 0;
+  int nsp = 0;
+  float height = 0.0f;
+  if (active) {
+    int k = A.next[i];
+    if (k < A.n_spikes[i] && (long long)A.time_idx[A.first[i] + k] == A.it) {
+      nsp = 1;
+      height = A.height[A.first[i] + k];
+      A.next[i] = k + 1;
+    }
+  }
+  emit_spikes(A.out, active, ii, A.i0 + ii, nsp, height);
+}
+
+// ---------------------------------------------------------------------
+// Delivery.  Pairs (spike, delay slot d) due at global step gs are the
+// spikes of ring slot gs-1-d combined with the source's synapse segment of
+// delay index d.  Wavefronts take tiles of 64 pairs; a wavefront scans the
+// 64 segment sizes, then its lanes stream the flattened synapse range of
+// the tile (consecutive lanes read consecutive 8-byte records) and add
+// height*weight into the target accumulators.
+// ---------------------------------------------------------------------
+constexpr int kMaxDelaySlots = 4096;
+
+template <bool PLASTIC>
+__global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
+  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
+  __shared__ uint32_t s_part[kBlock];
+  const int D = A.D;
+  const int tid = threadIdx.x;
+  // per-thread partial sums of the due spike counts over d
+  const int per = (D + kBlock - 1) / kBlock;
+  uint32_t run = 0;
+  for (int k = 0; k < per; k++) {
+    int d = tid * per + k;
+    uint32_t n = 0;
+    if (d < D) {
+      long long s = A.gs - 1 - d;
+      if (s >= 0) n = A.ring_n[s % A.R];
+    }
+    run += n;
+  }
+  s_part[tid] = run;
+  __syncthreads();
+  // inclusive scan of the block partials (Hillis-Steele in LDS)
+  for (int off = 1; off < kBlock; off <<= 1) {
+    uint32_t v = (tid >= off) ? s_part[tid - off] : 0u;
+    __syncthreads();
+    s_part[tid] += v;
+    __syncthreads();
+  }
+  {
+    uint32_t acc = (tid > 0) ? s_part[tid - 1] : 0u;
+    for (int k = 0; k < per; k++) {
+      int d = tid * per + k;
+      if (d < D) {
+        s_pre[d] = acc;
+        long long s = A.gs - 1 - d;
+        if (s >= 0) acc += A.ring_n[s % A.R];
+      }
+    }
+    if (tid == kBlock - 1) s_pre[D] = s_part[kBlock - 1];
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid == 0) A.ring_n[(A.gs + 1) % A.R] = 0u;
+  const uint32_t total = s_pre[D];
+  const uint32_t n_tiles = (total + 63u) / 64u;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (uint32_t)(blockIdx.x * (kBlock / 64) + tid / 64);
+  const uint32_t n_waves = gridDim.x * (kBlock / 64);
+  unsigned long long n_events = 0;
+  for (uint32_t wt = wave; wt < n_tiles; wt += n_waves) {
+    uint32_t pidx = wt * 64u + lane;
+    uint32_t cnt = 0, mult = 0;
+    unsigned long long start = 0;
+    float height = 0.0f;
+    if (pidx < total) {
+      // delay slot d of this pair: last d with s_pre[d] <= pidx
+      int lo = 0, hi = D;  // s_pre[lo] <= pidx < s_pre[hi]
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= pidx) lo = mid; else hi = mid;
+      }
+      int d = lo;
+      long long slot = (A.gs - 1 - d) % A.R;
+      long long e = slot * A.cap + (pidx - s_pre[d]);
+      uint32_t src = A.ring_src[e];
+      height = A.ring_h[e];
+      mult = A.ring_cnt[e];
+      const uint32_t *sg = A.seg + (size_t)src * (D + 1) + d;
+      uint32_t o0 = sg[0], o1 = sg[1];
+      cnt = o1 - o0;
+      start = A.base[src] + o0;
+    }
+    // exclusive scan of cnt over the wavefront
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t v = __shfl_up(incl, off);
+      if ((int)lane >= off) incl += v;
+    }
+    uint32_t pre = incl - cnt;
+    uint32_t E = __shfl(incl, 63);
+    n_events += E;
+    for (uint32_t e0 = 0; e0 < E; e0 += 64u) {
+      uint32_t e = e0 + lane;
+      // owner lane j: last lane with pre[j] <= e
+      int j = 0;
+#pragma unroll
+      for (int stp = 32; stp > 0; stp >>= 1) {
+        int cand = j + stp;
+        uint32_t pv = __shfl(pre, cand);
+        if (cand < 64 && pv <= e) j = cand;
+      }
+      uint32_t pj = __shfl(pre, j);
+      uint32_t slo = __shfl((uint32_t)start, j);
+      uint32_t shi = __shfl((uint32_t)(start >> 32), j);
+      float hj = __shfl(height, j);
+      uint32_t mj = __shfl(mult, j);
+      if (e < E) {
+        unsigned long long syn =
+            (((unsigned long long)shi << 32) | slo) + (e - pj);
+        unsigned long long r = A.rec[syn];
+        uint32_t tp = (uint32_t)r;
+        float w = __uint_as_float((uint32_t)(r >> 32));
+        uint32_t tgt = tp & kTargetMask;
+        uint32_t port = tp >> kPortShift;
+        unsafeAtomicAdd(A.acc + (long long)port * A.n_total + tgt, hj * w);
+        if (PLASTIC) {
+          int sgp = A.syn_group[syn];
+          if (sgp > 0) {
+            int type = A.syn_type[sgp - 1];
+            const float *prm = A.stdp_param + 8 * (sgp - 1);
+            float wn = w;
+            for (uint32_t rep = 0; rep < mj; rep++) {
+              A.spike_time[syn] = (uint16_t)(A.T & 0xffff);
+              long long Dt_int = A.T - A.last_rev[tgt];
+              if (Dt_int > 0 && Dt_int < kMaxSynDt)
+                wn = dev_syn_update(type, wn, -A.dt * (float)Dt_int, prm);
+            }
+            if (wn != w)
+              reinterpret_cast<uint32_t *>(A.rec + syn)[1] = __float_as_uint(wn);
+          }
+        }
+      }
+    }
+  }
+  if (lane == 0 && n_events) {
+    atomicAdd(A.events + (blockIdx.x & (kCounterSlots - 1)), n_events);
+  }
+}
+
+// ---------------------------------------------------------------------
+// Post-synaptic STDP (src/rev_spike.cu:41-75): every node whose spike
+// reaches its dendrites this step updates its plastic incoming synapses.
+// One wavefront per candidate node; lanes stride over its inputs.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs A) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (uint32_t)(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);
+  const uint32_t n_waves = gridDim.x * (kBlock / 64);
+  for (int k = 0; k <= A.max_den; k++) {
+    long long s = A.gs - k;
+    if (s < 0) break;
+    long long slot = s % A.R;
+    uint32_t n = A.ring_n[slot];
+    for (uint32_t e = wave; e < n; e += n_waves) {
+      uint32_t node = A.ring_src[slot * A.cap + e];
+      if (A.den_idx[node] != k) continue;
+      if (A.last_rev[node] != A.T) continue;
+      unsigned long long r0 = A.rev_ptr[node], r1 = A.rev_ptr[node + 1];
+      uint16_t ti = (uint16_t)(A.T & 0xffff);
+      for (unsigned long long r = r0 + lane; r < r1; r += 64) {
+        uint32_t syn = A.rev_idx[r];
+        int sgp = A.syn_group[syn];
+        if (sgp <= 0) continue;
+        uint16_t st = A.spike_time[syn];
+        uint16_t Dt_int = (uint16_t)(ti - st);
+        if (Dt_int < kMaxSynDt) {
+          uint32_t *wp = reinterpret_cast<uint32_t *>(A.rec + syn) + 1;
+          float w = __uint_as_float(*wp);
+          float wn = dev_syn_update(A.syn_type[sgp - 1], w,
+                                    A.dt * (float)Dt_int,
+                                    A.stdp_param + 8 * (sgp - 1));
+          *wp = __float_as_uint(wn);
+        }
+      }
+    }
+  }
+}
+
+__global__ void record_kernel(RecordArgs A) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A.n_col) return;
+  float v;
+  float *p = A.ptr[c];
+  if (p) {
+    v = *p;
+  } else {
+    int node = A.node[c];
+    v = (A.last_spike[node] == A.T) ? A.last_height[node] : 0.0f;
+  }
+  A.out[c] = v;
+}
+
+__global__ void wrap_kernel(uint16_t *st, unsigned long long n, int up) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint16_t v = st[i];
+  if (up) {
+    if (v >= 0x8000) st[i] = 0;
+  } else {
+    if (v < 0x8000) st[i] = 0x8000;
+  }
+}
+
+__global__ void fill_u16_kernel(uint16_t *p, unsigned long long n, uint16_t v) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <int P>
+void launch_aeif_p(const AeifArgs &a, hipStream_t s) {
+  int grid = (a.n + kBlock - 1) / kBlock;
+  if (grid > 0) hipLaunchKernelGGL(aeif_kernel<P>, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+}  // namespace
+
+void LaunchAeif(const AeifArgs &a, int n_port, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  switch (n_port) {
+#define NGPU_CASE(P) \
+  case P: launch_aeif_p<P>(a, s); break;
+    NGPU_CASE(1) NGPU_CASE(2) NGPU_CASE(3) NGPU_CASE(4) NGPU_CASE(5)
+    NGPU_CASE(6) NGPU_CASE(7) NGPU_CASE(8) NGPU_CASE(9) NGPU_CASE(10)
+    NGPU_CASE(11) NGPU_CASE(12) NGPU_CASE(13) NGPU_CASE(14) NGPU_CASE(15)
+    NGPU_CASE(16) NGPU_CASE(17) NGPU_CASE(18) NGPU_CASE(19) NGPU_CASE(20)
+#undef NGPU_CASE
+    default: break;
+  }
+}
+
+void LaunchIaf(const IafArgs &a, void *stream) {
+  int grid = (a.n + kBlock - 1) / kBlock;
+  if (grid > 0)
+    hipLaunchKernelGGL(iaf_kernel, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+}
+
+void LaunchRelay(const RelayArgs &a, void *stream) {
+  int grid = (a.n + kBlock - 1) / kBlock;
+  if (grid > 0)
+    hipLaunchKernelGGL(relay_kernel, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+}
+
+void LaunchSpikeGen(const SpikeGenArgs &a, void *stream) {
+  int grid = (a.n + kBlock - 1) / kBlock;
+  if (grid > 0)
+    hipLaunchKernelGGL(spikegen_kernel, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+}
+
+void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream) {
+  if (a.syn_group)
+    hipLaunchKernelGGL(delivery_kernel<true>, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(delivery_kernel<false>, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+}
+
+void LaunchRevStdp(const RevArgs &a, int grid, void *stream) {
+  hipLaunchKernelGGL(rev_kernel, dim3(grid), dim3(kBlock), 0,
+                     (hipStream_t)stream, a);
+}
+
+void LaunchRecord(const RecordArgs &a, void *stream) {
+  int grid = (a.n_col + 255) / 256;
+  if (grid > 0)
+    hipLaunchKernelGGL(record_kernel, dim3(grid), dim3(256), 0,
+                       (hipStream_t)stream, a);
+}
+
+void LaunchSpikeTimeWrap(uint16_t *spike_time, unsigned long long n, int up,
+                         void *stream) {
+  unsigned long long grid = (n + 255) / 256;
+  if (grid > 0)
+    hipLaunchKernelGGL(wrap_kernel, dim3((unsigned)grid), dim3(256), 0,
+                       (hipStream_t)stream, spike_time, n, up);
+}
+
+void LaunchFillU16(uint16_t *p, unsigned long long n, uint16_t v, void *stream) {
+  unsigned long long grid = (n + 255) / 256;
+  if (grid > 0)
+    hipLaunchKernelGGL(fill_u16_kernel, dim3((unsigned)grid), dim3(256), 0,
+                       (hipStream_t)stream, p, n, v);
+}
+
+}  // namespace ngpu

@@ -1,0 +1,130 @@
+// Node model descriptors: variable/parameter names in the order of the
+// reference enums, and NodeInit defaults.
+//   aeif_cond_beta   src/aeif_cond_beta_kernel.h:30-112, aeif_cond_beta.cu:25-56
+//   aeif_cond_alpha  src/aeif_cond_alpha_kernel.h:30-112, aeif_cond_alpha.cu:25-55
+//   iaf_psc_exp      src/iaf_psc_exp.h:32-83, iaf_psc_exp.cu:134-170
+//   spike_generator  src/spike_generator.cu:26-35
+//   poisson_generator src/poiss_gen_variables.h, src/poiss_gen.cu:77-95
+//   spike_detector   src/spike_detector.cu:26-38, 72-80
+//   parrot_neuron    src/parrot_neuron.cu:26-46, 75-100
+// The group parameter "rk5_tol" (RK5 error tolerance) has no reference
+// counterpart: the reference hard-codes 1e-6 (src/rk5.cu:55); its default
+// here is the same 1e-6.
+#include <algorithm>
+
+#include "engine.h"
+
+namespace ngpu {
+
+namespace {
+
+std::vector<ModelDesc> MakeModels() {
+  std::vector<ModelDesc> m(kNumModels);
+  ModelDesc &b = m[kAeifCondBeta];
+  b.name = "aeif_cond_beta";
+  b.scal_var = {"V_m", "w"};
+  b.port_var = {"g", "g1"};
+  b.scal_param = {"V_th",    "Delta_T", "g_L",    "E_L",   "C_m",
+                  "a",       "b",       "tau_w",  "I_e",   "V_peak",
+                  "V_reset", "t_ref",   "refractory_step", "den_delay"};
+  b.port_param = {"E_rev", "tau_rise", "tau_decay", "g0"};
+  b.group_param = {"h_min_rel", "h0_rel", "rk5_tol"};
+  b.scal_param_default = {-50.4f, 2.0f, 30.0f, -70.6f, 281.0f, 4.0f, 80.5f,
+                          144.0f, 0.0f,  0.0f, -60.0f, 0.0f,   0.0f, 0.0f};
+  b.port_param_default = {0.0f, 2.0f, 20.0f, 0.0f};
+  b.group_param_default = {1.0e-3f, 1.0e-2f, 1.0e-6f};
+  b.scal_var_default = {-70.6f, 0.0f};
+  b.port_var_default = {0.0f, 0.0f};
+  b.fixed_ports = -1;
+
+  ModelDesc &a = m[kAeifCondAlpha];
+  a = b;
+  a.name = "aeif_cond_alpha";
+  a.port_param = {"E_rev", "tau_syn", "g0"};
+  a.port_param_default = {0.0f, 2.0f, 0.0f};
+
+  ModelDesc &f = m[kIafPscExp];
+  f.name = "iaf_psc_exp";
+  f.scal_var = {"I_syn_ex", "I_syn_in", "V_m_rel", "refractory_step"};
+  f.scal_param = {"tau_m",   "C_m",    "E_L",   "I_e",   "Theta_rel",
+                  "V_reset_rel", "tau_ex", "tau_in", "t_ref", "den_delay",
+                  "P20",     "P11ex",  "P11in", "P21ex", "P21in", "P22"};
+  f.scal_param_default = {10.0f, 250.0f, -70.0f, 0.0f, 15.0f, 0.0f,
+                          2.0f,  2.0f,   2.0f,   0.0f, 0.0f,  0.0f,
+                          0.0f,  0.0f,   0.0f,   0.0f};
+  f.scal_var_default = {0.0f, 0.0f, 0.0f, 0.0f};
+  f.fixed_ports = 2;
+
+  ModelDesc &s = m[kSpikeGenerator];
+  s.name = "spike_generator";
+  s.array_param = {"spike_times", "spike_heights"};
+  s.fixed_ports = 0;
+
+  ModelDesc &p = m[kPoissonGenerator];
+  p.name = "poisson_generator";
+  p.scal_param = {"rate", "origin", "start", "stop"};
+  p.scal_param_default = {0.0f, 0.0f, 0.0f, 1.0e30f};
+  p.fixed_ports = 0;
+
+  ModelDesc &d = m[kSpikeDetector];
+  d.name = "spike_detector";
+  d.scal_var = {"input_spike_height", "spike_height"};
+  d.scal_param = {"hold_spike_height"};
+  d.scal_param_default = {1.0f};
+  d.scal_var_default = {0.0f, 0.0f};
+  d.fixed_ports = 1;
+
+  ModelDesc &r = m[kParrotNeuron];
+  r.name = "parrot_neuron";
+  r.scal_var = {"input_spike_height", "dummy_input", "V"};
+  r.scal_param = {"hold_spike_height", "den_delay"};
+  r.scal_param_default = {0.0f, 0.0f};
+  r.scal_var_default = {0.0f, 0.0f, 0.0f};
+  r.fixed_ports = 2;
+  return m;
+}
+
+}  // namespace
+
+const ModelDesc &Model(ModelId id) {
+  static const std::vector<ModelDesc> models = MakeModels();
+  return models[id];
+}
+
+int FindModel(const std::string &name) {
+  for (int i = 0; i < kNumModels; i++)
+    if (name == Model((ModelId)i).name) return i;
+  return -1;
+}
+
+int Group::VarIdx(const std::string &name, int port) const {
+  const ModelDesc &m = Model(model);
+  auto it = std::find(m.scal_var.begin(), m.scal_var.end(), name);
+  if (it != m.scal_var.end()) return (int)(it - m.scal_var.begin());
+  auto jt = std::find(m.port_var.begin(), m.port_var.end(), name);
+  if (jt != m.port_var.end()) {
+    if (port < 0 || port >= n_port) throw ngpu_exception("Port index out of range");
+    return (int)m.scal_var.size() + (int)m.port_var.size() * port +
+           (int)(jt - m.port_var.begin());
+  }
+  return -1;
+}
+
+int Group::ParamIdx(const std::string &name, int port) const {
+  const ModelDesc &m = Model(model);
+  auto it = std::find(m.scal_param.begin(), m.scal_param.end(), name);
+  if (it != m.scal_param.end()) return (int)(it - m.scal_param.begin());
+  auto jt = std::find(m.port_param.begin(), m.port_param.end(), name);
+  if (jt != m.port_param.end()) {
+    if (port < 0 || port >= n_port) throw ngpu_exception("Port index out of range");
+    return (int)m.scal_param.size() + (int)m.port_param.size() * port +
+           (int)(jt - m.port_param.begin());
+  }
+  return -1;
+}
+
+Group::~Group() {
+  for (auto *b : d_derived) delete b;
+}
+
+}  // namespace ngpu

@@ -1,0 +1,202 @@
+// Device-side data layout and kernel entry points of the per-timestep path.
+// Shared by the host engine (engine.cpp) and the HIP kernels (*.hip).
+//
+// Layout in HBM (one GPU, all resident):
+//   node state     SoA float arrays per group: var[v][n], plus x (double),
+//                  h (float), refractory (float) per RK5 neuron
+//   accumulators   acc[port][n_total] float, cleared by the consumer kernel
+//   spike ring     R slots x n_total entries: src (u32), height (f32),
+//                  count (u32); per-slot counter ring_n[R]
+//   synapses       CSR by source, delay-sorted within source:
+//                  rec[s] = {u32 target | port << 26, f32 weight} (8 B)
+//                  base[src] (u64) + seg[src][0..D] (u32, relative)
+//   direct (Poisson) connections, CSR by target: dir_ptr[n_total + 1]
+#pragma once
+#include <stdint.h>
+
+namespace ngpu {
+
+constexpr int kMaxPort = 20;          // src/aeif_cond_beta.h:26
+constexpr int kPortShift = 26;        // target in low 26 bits, port above
+constexpr uint32_t kTargetMask = (1u << kPortShift) - 1u;
+constexpr int kPoissonTableMax = 256;
+constexpr int kMaxSynDt = 16384;      // src/syn_model.h:21
+constexpr long long kLastSpikeTimeGuard = 0x70000000LL;
+constexpr int kMaxAttemptsPerStep = 100000;
+constexpr int kCounterSlots = 1024;   // spread statistic counters over L2
+constexpr uint32_t kPoissonTag = 0x504F4953u;
+
+// A parameter that is either uniform over a group (val) or per neuron (ptr).
+struct PField {
+  const float *ptr;
+  float val;
+};
+
+struct AeifParams {
+  PField V_th, Delta_T, inv_Delta_T, g_L, E_L, inv_C_m, a, b, inv_tau_w, I_e,
+      V_peak, V_reset, t_ref;
+  PField E_rev[kMaxPort], inv_tau_rise[kMaxPort], inv_tau_decay[kMaxPort],
+      g0[kMaxPort];
+};
+
+// Poisson direct connections feeding the targets of one group.
+struct DirectIn {
+  const uint32_t *ptr;        // [n_total+1] CSR by target (global node)
+  const uint32_t *canon;      // canonical connection index (Philox counter)
+  const uint32_t *gen;        // generator (global node) of the connection
+  const float *weight;
+  const float *delay;         // ms, dt*(d_int+1)
+  const uint8_t *port;
+  const float *gen_origin, *gen_start, *gen_stop;  // per global node
+  const uint32_t *gen_table;  // per global node: table index
+  const uint32_t *tables;     // [n_table][kPoissonTableMax]
+  uint32_t key0, key1;
+  int active;                 // any direct connections at all
+};
+
+// Inputs to fold at the start of a node update (end-of-step port input of
+// the previous step, src/get_spike.cu:116-140 and src/poiss_gen.cu:43-74).
+struct FoldIn {
+  float *acc;                 // [max_port][n_total]
+  long long n_total;
+  long long pois_T;           // time index of the pending Poisson draw, <0 none
+  double pois_t;              // neural time (ms) of that step
+  DirectIn dir;
+};
+
+struct SpikeOut {
+  uint32_t *ring_src;         // this step's slot
+  float *ring_h;
+  uint32_t *ring_cnt;
+  uint32_t *ring_n;           // counter of this step's slot
+  long long *last_spike;      // [n_total]
+  float *last_height;
+  long long *last_rev;
+  const int *den_idx;         // [n_total] dendritic delay in steps
+  int *spike_count;           // per group, nullptr if off
+  float *rec_times;           // per group, nullptr if off
+  int *n_rec_times;
+  int max_rec_times;
+  long long T;
+  double t;
+};
+
+struct AeifArgs {
+  int n, i0;
+  float *V, *w, *g[kMaxPort], *g1[kMaxPort];
+  double *x;
+  float *h;
+  float *refr;
+  AeifParams p;
+  float h_min;
+  double h_min_acc;           // h_min * (1 + 1e-5)
+  float eps;
+  float dt;
+  double x1;                  // end of this step (neural time, ms)
+  FoldIn fold;
+  SpikeOut out;
+  unsigned long long *cnt;    // [kCounterSlots][2] accepted, rejected
+  int do_step;                // 0: fold only (end-of-simulation flush)
+};
+
+struct IafArgs {
+  int n, i0;
+  float *I_ex, *I_in, *V, *refr;
+  PField I_e, Theta, V_reset, t_ref, P20, P11ex, P11in, P21ex, P21in, P22;
+  float dt;
+  FoldIn fold;
+  SpikeOut out;
+  int do_step;
+};
+
+// spike_detector (src/spike_detector.cu:46-68) and parrot_neuron
+// (src/parrot_neuron.cu:48-70): one input port folded into
+// input_spike_height.
+struct RelayArgs {
+  int n, i0;
+  int parrot;
+  float *input, *out_var;     // detector: spike_height; parrot: V
+  PField hold;
+  FoldIn fold;
+  SpikeOut out;
+  int do_step;
+};
+
+struct SpikeGenArgs {
+  int n, i0;
+  const int *n_spikes, *first;  // per node
+  const int *time_idx;          // flattened spike time indexes
+  const float *height;
+  int *next;
+  long long it;                 // step index within this Simulate call
+  SpikeOut out;
+};
+
+struct DeliveryArgs {
+  const uint32_t *ring_src;     // [R][cap]
+  const float *ring_h;
+  const uint32_t *ring_cnt;
+  uint32_t *ring_n;             // [R]
+  long long cap;
+  int R;
+  long long gs;                 // global step index of this step
+  int D;                        // number of delay slots (max d_int + 1)
+  const unsigned long long *base;  // [n_total+1]
+  const uint32_t *seg;          // [n_total][D+1]
+  unsigned long long *rec;      // synapse records, weight in high word
+  float *acc;
+  long long n_total;
+  // plasticity
+  const uint8_t *syn_group;     // nullptr if no plastic synapse
+  uint16_t *spike_time;
+  const long long *last_rev;
+  const float *stdp_param;      // [n_syn_group][8]
+  const int *syn_type;          // [n_syn_group]
+  long long T;
+  float dt;
+  unsigned long long *events;   // [kCounterSlots]
+};
+
+struct RevArgs {
+  const uint32_t *ring_src;
+  long long cap;
+  int R;
+  long long gs;
+  const uint32_t *ring_n;
+  const int *den_idx;
+  int max_den;
+  const unsigned long long *rev_ptr;  // [n_total+1]
+  const uint32_t *rev_idx;            // synapse index
+  unsigned long long *rec;
+  const uint8_t *syn_group;
+  const uint16_t *spike_time;
+  const float *stdp_param;
+  const int *syn_type;
+  const long long *last_rev;
+  long long T;
+  float dt;
+};
+
+struct RecordArgs {
+  int n_col;
+  float *const *ptr;            // per column; nullptr = spike activity
+  const int *node;              // per column (global node) for spikes
+  const long long *last_spike;
+  const float *last_height;
+  long long T;
+  float *out;                   // row base
+};
+
+// launchers (kernels.hip); all asynchronous on `stream`
+void LaunchAeif(const AeifArgs &a, int n_port, void *stream);
+void LaunchIaf(const IafArgs &a, void *stream);
+void LaunchRelay(const RelayArgs &a, void *stream);
+void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
+void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
+void LaunchRevStdp(const RevArgs &a, int grid, void *stream);
+void LaunchRecord(const RecordArgs &a, void *stream);
+void LaunchSpikeTimeWrap(uint16_t *spike_time, unsigned long long n, int up,
+                         void *stream);
+void LaunchFillU16(uint16_t *p, unsigned long long n, uint16_t v, void *stream);
+
+}  // namespace ngpu

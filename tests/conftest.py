@@ -1,0 +1,26 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+
+def pytest_configure(config):
+    config.addinivalue_line(
+        "markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+@pytest.fixture
+def ngpu():
+    """The engine's Python bindings, reset to a fresh kernel."""
+    from neurongpu_amd import neurongpu as ng
+    ng.ResetKernel()
+    ng.SetVerbosityLevel(0)
+    yield ng
+    ng.ResetKernel()

@@ -1,0 +1,216 @@
+"""Builds one network description in both the engine (through the Python
+bindings, i.e. the C ABI) and the CPU oracle, so GPU and oracle runs see the
+same nodes, parameters, synapses and Poisson streams.
+
+A description is a dict:
+  dt, seed, groups: [ {model, n, n_port, params{name: scalar|list|array},
+                       vars{...}, rk5: {h_min_rel, h0_rel, rk5_tol}} ],
+  conns: [ {src, tgt, weight, delay, port, syn_group} ]   (arrays, one entry
+          per synapse, connected one_to_one in the given order)
+  poisson: [ {node, rate, origin, start, stop} ]
+  spike_gen: [ {node, times, heights} ]
+  stdp: [ {param dict} ]       (syn groups 1..)
+  records: [ (node, var, port) ]
+"""
+import numpy as np
+
+from oracle import pyoracle as po
+
+PORT_PARAMS = {"E_rev", "tau_rise", "tau_decay", "tau_syn", "g0"}
+
+
+def build_engine(ngpu, spec):
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetTimeResolution(spec.get("dt", 0.1))
+    ngpu.SetRandomSeed(spec.get("seed", 54321))
+    groups = []
+    for g in spec["groups"]:
+        ns = ngpu.Create(g["model"], g["n"], g.get("n_port", 1))
+        groups.append(ns)
+        for k, v in g.get("params", {}).items():
+            _set_engine(ngpu, ns, k, v)
+        for k, v in g.get("vars", {}).items():
+            _set_engine(ngpu, ns, k, v)
+        if "rk5" in g:
+            for k, v in g["rk5"].items():
+                ngpu.SetStatus(ns, {k: v})
+    for s in spec.get("stdp", []):
+        ngpu.CreateSynGroup("stdp", dict(s))
+    for p in spec.get("poisson", []):
+        ngpu.SetStatus([p["node"]], {"rate": p["rate"],
+                                     "origin": p.get("origin", 0.0),
+                                     "start": p.get("start", 0.0),
+                                     "stop": p.get("stop", 1.0e30)})
+    for sg in spec.get("spike_gen", []):
+        d = {"spike_times": list(map(float, sg["times"]))}
+        if sg.get("heights") is not None:
+            d["spike_heights"] = list(map(float, sg["heights"]))
+        ngpu.SetStatus([sg["node"]], d)
+    for c in spec.get("conns", []):
+        src = np.asarray(c["src"], dtype=np.int32)
+        n = src.size
+        syn = {"weight": {"array": np.broadcast_to(
+            np.asarray(c["weight"], np.float32), n)},
+               "delay": {"array": np.broadcast_to(
+                   np.asarray(c["delay"], np.float32), n)},
+               "receptor": int(c.get("port", 0)),
+               "synapse_group": int(c.get("syn_group", 0))}
+        ngpu.Connect(src, np.asarray(c["tgt"], dtype=np.int32),
+                     {"rule": "one_to_one"}, syn)
+    recs = []
+    for (node, var, port) in spec.get("records", []):
+        recs.append(ngpu.CreateRecord("", [var], [node], [port]))
+    return groups, recs
+
+
+def _set_engine(ngpu, ns, k, v):
+    if k in PORT_PARAMS:
+        ngpu.SetStatus(ns, {k: list(map(float, v))})
+    elif np.ndim(v) == 0:
+        ngpu.SetStatus(ns, {k: float(v)})
+    else:
+        ngpu.SetStatus(ns, {k: {"array": np.asarray(v, np.float32)}})
+
+
+def build_oracle(spec, prec="f32"):
+    net = po.Net(dt=spec.get("dt", 0.1), seed=spec.get("seed", 54321),
+                 prec=prec)
+    groups = []
+    for g in spec["groups"]:
+        og = net.create(g["model"], g["n"], g.get("n_port", 1))
+        groups.append(og)
+        for k, v in list(g.get("params", {}).items()) + list(
+                g.get("vars", {}).items()):
+            og.set(k, v)
+        if "rk5" in g:
+            r = {"h_min_rel": 1e-3, "h0_rel": 1e-2, "rk5_tol": 1e-6}
+            r.update(g["rk5"])
+            net.set_rk5(og, np.float32(r["h_min_rel"]),
+                        np.float32(r["h0_rel"]), np.float32(r["rk5_tol"]))
+    for s in spec.get("stdp", []):
+        net.stdp_group(**s)
+    for p in spec.get("poisson", []):
+        net.poisson(p["node"], p["rate"], p.get("origin", 0.0),
+                    p.get("start", 0.0), p.get("stop", 1.0e30))
+    for sg in spec.get("spike_gen", []):
+        net.spike_generator_times(sg["node"], sg["times"], sg.get("heights"))
+    for c in spec.get("conns", []):
+        net.connect(c["src"], c["tgt"], c["weight"], c["delay"],
+                    c.get("port", 0), c.get("syn_group", 0))
+    for (node, var, port) in spec.get("records", []):
+        net.record(node, var, port)
+    return net, groups
+
+
+def engine_spikes(ngpu, group):
+    """(node, time) pairs from ActivateRecSpikeTimes of a group."""
+    out = []
+    for i in range(group.n):
+        for t in ngpu.GetRecSpikeTimes(group[i]):
+            out.append((group[i], t))
+    return out
+
+
+def oracle_spikes(net, dt, node_lo, node_hi):
+    st, nd, ct = net.spikes()
+    out = []
+    for s, n, c in zip(st, nd, ct):
+        if node_lo <= n < node_hi:
+            # engine time stamp: t0 + (double)(float)dt * (step + 1)
+            t = np.float32(np.float64(np.float32(dt)) * (s + 1))
+            for _ in range(c):
+                out.append((int(n), float(t)))
+    return out
+
+
+def c0_spec(n=1000, k_e=80, k_i=20, w_e=0.05, w_i=0.25, w_p=0.1,
+            rate=8000.0, seed=1234, tol=1e-3, n_port=2):
+    """Plain-method network of the small configuration: aeif_cond_beta, 2
+    receptor ports (E_rev 0/-85 mV, tau_rise 0.2/0.5 ms, tau_decay 2/10 ms),
+    fixed indegree k_e + k_i without replacement, integer delays 1-20
+    steps, own Poisson generator per neuron."""
+    rng = np.random.default_rng(seed)
+    n_e = int(round(n * 0.8))
+    v0 = rng.uniform(-70.0, -55.0, n).astype(np.float32)
+    src, tgt, w, d, port = [], [], [], [], []
+    for j in range(n):
+        se = rng.choice(n_e, size=k_e, replace=False)
+        si = n_e + rng.choice(n - n_e, size=k_i, replace=False)
+        src += [se, si]
+        tgt += [np.full(k_e, j), np.full(k_i, j)]
+        w += [np.full(k_e, w_e), np.full(k_i, w_i)]
+        d += [rng.integers(1, 21, k_e), rng.integers(1, 21, k_i)]
+        port += [np.zeros(k_e, int), np.ones(k_i, int)]
+    src = np.concatenate(src)
+    tgt = np.concatenate(tgt)
+    w = np.concatenate(w).astype(np.float32)
+    d = (np.concatenate(d) * 0.1).astype(np.float32)
+    port = np.concatenate(port)
+    params = {"C_m": 281.0, "g_L": 30.0, "E_L": -70.6, "V_th": -50.4,
+              "Delta_T": 2.0, "a": 4.0, "tau_w": 144.0, "b": 80.5,
+              "V_reset": -60.0, "t_ref": 0.0,
+              "E_rev": [0.0, -85.0], "tau_rise": [0.2, 0.5],
+              "tau_decay": [2.0, 10.0]}
+    spec = {"dt": 0.1, "seed": seed,
+            "groups": [{"model": "aeif_cond_beta", "n": n, "n_port": n_port,
+                        "params": params, "vars": {"V_m": v0},
+                        "rk5": {"rk5_tol": tol}},
+                       {"model": "poisson_generator", "n": n}],
+            "conns": [], "poisson": []}
+    for p in (0, 1):
+        m = port == p
+        spec["conns"].append({"src": src[m], "tgt": tgt[m], "weight": w[m],
+                              "delay": d[m], "port": p})
+    spec["conns"].append({"src": np.arange(n, 2 * n), "tgt": np.arange(n),
+                          "weight": np.full(n, w_p, np.float32),
+                          "delay": np.full(n, 0.1, np.float32), "port": 0})
+    spec["poisson"] = [{"node": n + i, "rate": rate} for i in range(n)]
+    return spec
+
+
+def brunel_spec(order=200, seed=1234, w_ex=0.05, w_in=0.35, w_p=0.37,
+                rate=20000.0, tol=1e-6):
+    """The reference's network-statistics test (python/test/
+    test_brunel_list.py:14-85): 4*order excitatory + order inhibitory
+    aeif_cond_beta neurons, 2 ports (E_rev 0/-85 mV, tau_rise = tau_decay =
+    1 ms), CE = 800 excitatory and CI = 200 inhibitory inputs per neuron
+    (fixed indegree), delays normal N(0.5, 0.25) ms clipped to
+    [0.1, 1.25], one 20 kHz Poisson generator connected to every neuron
+    with weight 0.37 and delay 0.2 ms.  Its expected mean rate is 30.78
+    spikes/s (test_brunel_list.py:11)."""
+    rng = np.random.default_rng(seed)
+    n_e, n_i = 4 * order, order
+    n = n_e + n_i
+    ce, ci = 800, 200
+
+    def delays(m):
+        d = rng.normal(0.5, 0.25, m)
+        bad = (d < 0.1) | (d > 1.25)
+        while bad.any():
+            d[bad] = rng.normal(0.5, 0.25, bad.sum())
+            bad = (d < 0.1) | (d > 1.25)
+        return d.astype(np.float32)
+
+    src_e = np.concatenate([rng.choice(n_e, ce, replace=False)
+                            for _ in range(n)])
+    tgt_e = np.repeat(np.arange(n), ce)
+    src_i = np.concatenate([n_e + rng.choice(n_i, ci, replace=False)
+                            for _ in range(n)])
+    tgt_i = np.repeat(np.arange(n), ci)
+    params = {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0],
+              "tau_decay": [1.0, 1.0]}
+    return {"dt": 0.1, "seed": seed,
+            "groups": [{"model": "aeif_cond_beta", "n": n, "n_port": 2,
+                        "params": params, "rk5": {"rk5_tol": tol}},
+                       {"model": "poisson_generator", "n": 1}],
+            "conns": [{"src": src_e, "tgt": tgt_e,
+                       "weight": np.full(src_e.size, w_ex, np.float32),
+                       "delay": delays(src_e.size), "port": 0},
+                      {"src": src_i, "tgt": tgt_i,
+                       "weight": np.full(src_i.size, w_in, np.float32),
+                       "delay": delays(src_i.size), "port": 1},
+                      {"src": np.full(n, n), "tgt": np.arange(n),
+                       "weight": np.full(n, w_p, np.float32),
+                       "delay": np.full(n, 0.2, np.float32), "port": 0}],
+            "poisson": [{"node": n, "rate": rate}]}

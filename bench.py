@@ -1,0 +1,352 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-timestep spiking-network path on one MI355X.
+
+A step is one simulation time step (dt = 0.1 ms) of the whole network:
+the fused AdEx RK5 update (with the Poisson drive and the port-input fold
+of the previous step), spike-ring append and the synapse delivery pass.
+The timed region is `steps` steps run through NeuronGPU_Simulate, inputs
+and synapses already resident in HBM; the reported value is wall seconds
+per biological second.
+
+Workloads:
+  c1        1,000,000 aeif_cond_beta (800k E / 200k I), 2 ports
+            (E_rev 0/-85 mV, tau_rise 0.2/0.5, tau_decay 2/10 ms), fixed
+            indegree 1000 (800 E w=0.05 nS port 0, 200 I w=0.25 nS port 1),
+            delays uniform 1-20 steps, 8 kHz Poisson drive per neuron
+            (w=0.1 nS), V0 ~ U(-70,-55) mV, RK5 tolerance 1e-3, seed 1234.
+            This is the headline workload (`value`).  With these
+            parameters the Poisson drive stays below the AdEx rheobase and
+            the network is silent.
+  readme    the reference README's 1 M neuron / 1 B synapse benchmark network
+            (c++/examples/brunel_net.cpp at n_neurons = 1e6): 20 kHz
+            Poisson (w=0.37), tau_rise = tau_decay = 1 ms, Win = 0.35,
+            delays N(0.5, 0.25) ms clipped [0.1, 1.25], RK5 tolerance 1e-6.
+            An active network (~30 spikes/s); the reference quotes "little
+            more than 70 s per second of neural activity" on an RTX 2080 Ti
+            (README.md:11).  Reported in the same JSON line.
+
+CPU baseline: the FP32 CPU oracle (OpenMP over neurons) on a bounded
+sample of the c1 workload (reduced neuron count, same per-neuron
+indegree, weights, delays and drive), scaled linearly to 1 M neurons and
+1 s of biological time.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("wall s per bio s + synaptic events/s (1M AdEx, 1B syn); "
+          "RK5 stages/step learned vs plain")
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# algorithmic work per unit (RK5 aeif P=2, delivery)
+RK5_BYTES_PER_NEURON_STEP = 72.0   # 2*4*NVAR + 8 h + 8 refractory + 4*P
+RK5_FLOP_ACCEPTED = 512.0
+RK5_FLOP_REJECTED = 465.0
+DELIVERY_BYTES_PER_EVENT = 9.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hip_sync():
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipDeviceSynchronize()
+
+
+def build_c1(ngpu, n=1000000, seed=1234, tol=1e-3):
+    rng = np.random.default_rng(seed)
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetRandomSeed(seed)
+    n_e = int(round(0.8 * n))
+    neuron = ngpu.Create("aeif_cond_beta", n, 2)
+    ngpu.SetStatus(neuron, {
+        "C_m": 281.0, "g_L": 30.0, "E_L": -70.6, "V_th": -50.4,
+        "Delta_T": 2.0, "a": 4.0, "tau_w": 144.0, "b": 80.5,
+        "V_reset": -60.0, "t_ref": 0.0, "E_rev": [0.0, -85.0],
+        "tau_rise": [0.2, 0.5], "tau_decay": [2.0, 10.0]})
+    ngpu.SetStatus(neuron, {"rk5_tol": tol})
+    t = time.time()
+    v0 = rng.uniform(-70.0, -55.0, n).astype(np.float32)
+    ngpu.SetStatus(neuron, {"V_m": {"array": v0}})
+    log("  V0 set in %.1f s" % (time.time() - t))
+    pg = ngpu.Create("poisson_generator", 1)
+    ngpu.SetStatus(pg, {"rate": 8000.0})
+    exc = neuron[0:n_e]
+    inh = neuron[n_e:n]
+    k_e, k_i = 800, 200
+    t = time.time()
+    d_e = (rng.integers(1, 21, n * k_e, dtype=np.int8).astype(np.float32)
+           * np.float32(0.1))
+    ngpu.Connect(exc, neuron, {"rule": "fixed_indegree", "indegree": k_e},
+                 {"weight": 0.05, "delay": {"array": d_e}, "receptor": 0})
+    del d_e
+    d_i = (rng.integers(1, 21, n * k_i, dtype=np.int8).astype(np.float32)
+           * np.float32(0.1))
+    ngpu.Connect(inh, neuron, {"rule": "fixed_indegree", "indegree": k_i},
+                 {"weight": 0.25, "delay": {"array": d_i}, "receptor": 1})
+    del d_i
+    ngpu.Connect(pg, neuron, {"rule": "all_to_all"},
+                 {"weight": 0.1, "delay": 0.1, "receptor": 0})
+    log("  connections in %.1f s" % (time.time() - t))
+    return neuron, n * (k_e + k_i)
+
+
+def build_readme(ngpu, n=1000000, seed=1234):
+    """c++/examples/brunel_net.cpp:31-99 at n_neurons = n."""
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetRandomSeed(seed)
+    order = n // 5
+    n_e, n_i = 4 * order, order
+    ce, ci = 800, 200
+    pg = ngpu.Create("poisson_generator")
+    ngpu.SetStatus(pg, {"rate": 20000.0})
+    neuron = ngpu.Create("aeif_cond_beta", n_e + n_i, 2)
+    exc = neuron[0:n_e]
+    inh = neuron[n_e:n_e + n_i]
+    ngpu.SetStatus(neuron, {"E_rev": [0.0, -85.0], "tau_decay": [1.0, 1.0],
+                            "tau_rise": [1.0, 1.0]})
+    t = time.time()
+    d_e = ngpu.RandomNormalClipped(ce * (n_e + n_i), 0.5, 0.25, 0.1,
+                                   0.5 + 3 * 0.25)
+    ngpu.Connect(exc, neuron, {"rule": "fixed_indegree", "indegree": ce},
+                 {"weight": 0.05, "delay": {"array": d_e}, "receptor": 0})
+    del d_e
+    d_i = ngpu.RandomNormalClipped(ci * (n_e + n_i), 0.5, 0.25, 0.1,
+                                   0.5 + 3 * 0.25)
+    ngpu.Connect(inh, neuron, {"rule": "fixed_indegree", "indegree": ci},
+                 {"weight": 0.35, "delay": {"array": d_i}, "receptor": 1})
+    del d_i
+    ngpu.Connect(pg, neuron, {"rule": "all_to_all"},
+                 {"weight": 0.37, "delay": 0.2, "receptor": 0})
+    log("  connections in %.1f s" % (time.time() - t))
+    return neuron, (n_e + n_i) * (ce + ci)
+
+
+def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps):
+    dt = ngpu.GetTimeResolution()
+    ngpu.ActivateSpikeCount(neuron)
+    t = time.time()
+    ngpu.Calibrate()
+    log("  calibrate (CSR build + upload) in %.1f s" % (time.time() - t))
+    if warmup > 0:
+        ngpu.Simulate(warmup * dt)
+    s0 = ngpu.GetHotPathStats()
+    c0 = int(np.sum(ngpu.GetNeuronIntVar(neuron.i0, neuron.n,
+                                         "spike_count")))
+    hip_sync()
+    t0 = time.perf_counter()
+    ngpu.Simulate(steps * dt)
+    hip_sync()
+    wall = time.perf_counter() - t0
+    s1 = ngpu.GetHotPathStats()
+    c1 = int(np.sum(ngpu.GetNeuronIntVar(neuron.i0, neuron.n,
+                                         "spike_count")))
+    bio_s = steps * dt * 1e-3
+    events = s1["syn_events"] - s0["syn_events"]
+    acc = s1["rk5_accepted"] - s0["rk5_accepted"]
+    rej = s1["rk5_rejected"] - s0["rk5_rejected"]
+    spikes = c1 - c0
+    res = {
+        "wall_s": wall, "bio_s": bio_s, "wall_s_per_bio_s": wall / bio_s,
+        "ms_per_step": 1e3 * wall / steps, "syn_events": events,
+        "syn_events_per_s": events / wall,
+        "rate_hz": spikes / neuron.n / bio_s,
+        "rk5_accepted": acc, "rk5_rejected": rej,
+        "rk5_stages_per_neuron_step": (6.0 * acc + 5.0 * rej) /
+        (neuron.n * steps),
+    }
+    # per-kernel device time (HIP events on the engine stream)
+    ngpu.SetKernelTiming(1)
+    s2 = ngpu.GetHotPathStats()
+    ngpu.Simulate(timing_steps * dt)
+    kt = ngpu.GetKernelTimes()
+    s3 = ngpu.GetHotPathStats()
+    ngpu.SetKernelTiming(0)
+    res["kernel_ms"] = {k: {"total_ms": v[0], "launches": v[1],
+                            "avg_us": 1e3 * v[0] / max(v[1], 1)}
+                        for k, v in kt.items()}
+    acc_t = s3["rk5_accepted"] - s2["rk5_accepted"]
+    rej_t = s3["rk5_rejected"] - s2["rk5_rejected"]
+    ev_t = s3["syn_events"] - s2["syn_events"]
+    rk = kt.get("aeif_rk5", (0.0, 0))
+    dl = kt.get("delivery", (0.0, 0))
+    res["rk5_kernel"] = {
+        "avg_us": 1e3 * rk[0] / max(rk[1], 1),
+        "gbs": RK5_BYTES_PER_NEURON_STEP * neuron.n * rk[1] /
+        max(rk[0] * 1e-3, 1e-12) / 1e9,
+        "tflops": (RK5_FLOP_ACCEPTED * acc_t + RK5_FLOP_REJECTED * rej_t) /
+        max(rk[0] * 1e-3, 1e-12) / 1e12}
+    res["delivery_kernel"] = {
+        "avg_us": 1e3 * dl[0] / max(dl[1], 1),
+        "events_per_launch": ev_t / max(dl[1], 1),
+        "gbs": DELIVERY_BYTES_PER_EVENT * ev_t / max(dl[0] * 1e-3, 1e-12)
+        / 1e9}
+    res["n_neurons"] = neuron.n
+    res["n_synapses"] = n_syn
+    return res
+
+
+def roofline(res):
+    """Dominant kernel of the step and its algorithmic traffic."""
+    km = res["kernel_ms"]
+    dom = max(km, key=lambda k: km[k]["total_ms"])
+    if dom == "delivery":
+        per_launch = DELIVERY_BYTES_PER_EVENT * res["delivery_kernel"][
+            "events_per_launch"]
+        ach = res["delivery_kernel"]["gbs"]
+    else:
+        per_launch = RK5_BYTES_PER_NEURON_STEP * res["n_neurons"]
+        ach = res["rk5_kernel"]["gbs"]
+    return {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": per_launch,
+            "avg_launch_us": round(km[dom]["avg_us"], 2),
+            "traffic": None}
+
+
+def cpu_baseline(n_sample=100000, steps=500, warmup=100, seed=1234):
+    """FP32 oracle (OpenMP over neurons) on a c1 sample."""
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(seed)
+    n = n_sample
+    n_e = int(round(0.8 * n))
+    k_e, k_i = 800, 200
+    net = po.Net(dt=0.1, seed=seed, prec="f32")
+    g = net.create("aeif_cond_beta", n, 2)
+    for k, v in {"C_m": 281.0, "g_L": 30.0, "E_L": -70.6, "V_th": -50.4,
+                 "Delta_T": 2.0, "a": 4.0, "tau_w": 144.0, "b": 80.5,
+                 "V_reset": -60.0, "t_ref": 0.0}.items():
+        g.set(k, v)
+    g.set("E_rev", [0.0, -85.0])
+    g.set("tau_rise", [0.2, 0.5])
+    g.set("tau_decay", [2.0, 10.0])
+    g.set("V_m", rng.uniform(-70.0, -55.0, n).astype(np.float32))
+    net.set_rk5(g, np.float32(1e-3), np.float32(1e-2), np.float32(1e-3))
+    pg = net.create("poisson_generator", 1)
+    net.poisson(pg.i0, 8000.0)
+    tgt_e = np.repeat(np.arange(n, dtype=np.int32), k_e)
+    src_e = rng.integers(0, n_e, n * k_e).astype(np.int32)
+    net.connect(src_e, tgt_e, 0.05,
+                (rng.integers(1, 21, n * k_e) * 0.1).astype(np.float32), 0)
+    del src_e, tgt_e
+    tgt_i = np.repeat(np.arange(n, dtype=np.int32), k_i)
+    src_i = (n_e + rng.integers(0, n - n_e, n * k_i)).astype(np.int32)
+    net.connect(src_i, tgt_i, 0.25,
+                (rng.integers(1, 21, n * k_i) * 0.1).astype(np.float32), 1)
+    del src_i, tgt_i
+    net.connect(np.full(n, pg.i0, np.int32), np.arange(n, dtype=np.int32),
+                0.1, 0.1, 0)
+    net.calibrate()
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    net.run(warmup, cores)
+    t0 = time.perf_counter()
+    net.run(steps, cores)
+    wall = time.perf_counter() - t0
+    bio_s = steps * 0.1e-3
+    scaled = wall / bio_s * (1.0e6 / n)
+    return {"value": round(scaled, 2), "unit": "wall-s/bio-s",
+            "cores": cores, "kind": "port",
+            "sample": ("FP32 CPU oracle, c1 workload at %d neurons "
+                       "(indegree 800+200, same weights/delays/drive; sources"
+                       " drawn with replacement), %d steps (%.0f ms bio) "
+                       "after %d warm-up steps, %.2f s wall, scaled x%.0f "
+                       "in neurons to 1e6 and to 1 s bio"
+                       % (n, steps, bio_s * 1e3, warmup, wall, 1.0e6 / n))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--neurons", type=int, default=1000000)
+    ap.add_argument("--timing-steps", type=int, default=300)
+    ap.add_argument("--no-readme", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-neurons", type=int, default=100000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU "
+              "path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from neurongpu_amd import neurongpu as ngpu
+
+    log("building c1 (%d neurons)" % args.neurons)
+    neuron, n_syn = build_c1(ngpu, args.neurons)
+    c1 = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
+                 args.timing_steps)
+    log("c1:", json.dumps(c1))
+    out = {
+        "metric": METRIC,
+        "value": round(c1["wall_s_per_bio_s"], 4),
+        "unit": "wall-s/bio-s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(c1["ms_per_step"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded network, Philox Poisson drive)",
+        "config": {"workload": "c1: 1M aeif_cond_beta, 1e9 synapses "
+                               "(800 E w=0.05, 200 I w=0.25), delays 1-20 "
+                               "steps, 8 kHz Poisson w=0.1, RK5 tol 1e-3, "
+                               "dt 0.1 ms",
+                   "neurons": c1["n_neurons"], "synapses": c1["n_synapses"]},
+        "syn_events_per_s": round(c1["syn_events_per_s"], 1),
+        "rate_hz": round(c1["rate_hz"], 3),
+        "rk5_stages_per_neuron_step": round(
+            c1["rk5_stages_per_neuron_step"], 4),
+        "kernels": c1["kernel_ms"],
+        "rk5_valu": {"achieved": round(c1["rk5_kernel"]["tflops"], 3),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(c1["rk5_kernel"]["tflops"] /
+                                   FP32_PEAK_TFLOPS, 4)},
+        "roofline": roofline(c1),
+    }
+    if not args.no_readme:
+        ngpu.ResetKernel()
+        log("building readme network (%d neurons)" % args.neurons)
+        neuron, n_syn = build_readme(ngpu, args.neurons)
+        rd = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
+                     args.timing_steps)
+        log("readme:", json.dumps(rd))
+        out["readme_network"] = {
+            "workload": "reference README benchmark network "
+                        "(c++/examples/brunel_net.cpp, 1M neurons, 1e9 "
+                        "synapses, 20 kHz Poisson, RK5 tol 1e-6)",
+            "wall_s_per_bio_s": round(rd["wall_s_per_bio_s"], 4),
+            "ms_per_step": round(rd["ms_per_step"], 4),
+            "syn_events_per_s": round(rd["syn_events_per_s"], 1),
+            "rate_hz": round(rd["rate_hz"], 3),
+            "rk5_stages_per_neuron_step": round(
+                rd["rk5_stages_per_neuron_step"], 4),
+            "reference_wall_s_per_bio_s": 70.0,
+            "reference_hardware": "RTX 2080 Ti (README.md:11)",
+            "speedup_vs_reference": round(70.0 / rd["wall_s_per_bio_s"], 2),
+            "kernels": rd["kernel_ms"],
+            "roofline": roofline(rd)}
+        ngpu.ResetKernel()
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_neurons)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -396,6 +396,11 @@ class NeuronGPU {
 
   // no reference counterpart: hot-path statistics of the runs so far
   NeuronGPUStats GetStats();
+  // no reference counterpart: per-kernel device time with HIP events
+  // (kinds 0 AdEx RK5, 1 iaf_psc_exp, 2 detector/parrot, 3 spike
+  // generator, 4 delivery, 5 post-synaptic STDP, 6 record)
+  int SetKernelTiming(int on);
+  int GetKernelTimes(double *ms8, long long *cnt8);
 };
 
 #endif

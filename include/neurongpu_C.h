@@ -190,6 +190,13 @@ extern "C" {
    * out[1] RK5 rejected attempts, out[2] synaptic events delivered. */
   int NeuronGPU_GetHotPathStats(unsigned long long *out3);
 
+  /* Per-kernel device time measured with HIP events on the engine's stream
+   * (on != 0 enables; enabling or disabling clears the totals).  Kinds:
+   * 0 AdEx RK5 update, 1 iaf_psc_exp update, 2 spike detector / parrot,
+   * 3 spike generator, 4 delivery, 5 post-synaptic STDP, 6 record. */
+  int NeuronGPU_SetKernelTiming(int on);
+  int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,28 @@
+// On-device build of the synapse store (build_store.hip).
+#pragma once
+#include <stdint.h>
+
+#include "ngpu_kernels.h"
+
+namespace ngpu {
+
+// Sorts S connections (insertion order) into the canonical store.  All
+// pointers are device pointers; sg_in/sg_out may be null (no plasticity).
+// Returns null on success or a static error message.
+const char *BuildSynapseStore(const uint32_t *src, const uint32_t *tp,
+                              const float *w, const uint16_t *d,
+                              const uint8_t *sg_in, unsigned long long S,
+                              uint32_t N, uint32_t D,
+                              unsigned long long *rec, uint8_t *sg_out,
+                              unsigned long long *base, uint32_t *seg,
+                              void *stream);
+
+// Plastic synapses grouped by target in canonical synapse order.
+// rev_ptr (device, N+1) is written; *rev_idx_out is a new device array.
+const char *BuildReverseIndex(const unsigned long long *rec,
+                              const uint8_t *sg, unsigned long long S,
+                              uint32_t N, unsigned long long *rev_ptr,
+                              uint32_t **rev_idx_out,
+                              unsigned long long *n_rev_out, void *stream);
+
+}  // namespace ngpu

@@ -1,0 +1,191 @@
+// On-device build of the synapse store at calibration.
+//
+// Input: connections in insertion order (coordinate form) already in HBM.
+// Output: the canonical store the delivery kernel reads: synapses sorted by
+// (source, delay index) with insertion order kept inside a delay group
+// (the grouping of src/connect.cu:44-62 and the flattening of
+// src/spike_buffer.cu:381-413), as 8-byte records {target|port<<26,
+// weight}, plus per-source base offsets and per-(source, delay) segment
+// offsets.  A stable LSD radix sort on key = source*D + delay with the
+// insertion index as value gives exactly that order; a histogram + scan over
+// the keys gives the segment offsets.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "build_store.h"
+
+namespace ngpu {
+
+namespace {
+
+__global__ void make_keys(const uint32_t *src, const uint16_t *d, uint32_t D,
+                          unsigned long long S, uint32_t *keys,
+                          uint32_t *vals, uint32_t *count) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  uint32_t k = src[i] * D + d[i];
+  keys[i] = k;
+  vals[i] = (uint32_t)i;
+  atomicAdd(count + k, 1u);
+}
+
+__global__ void gather_records(const uint32_t *vals, const uint32_t *tp,
+                               const float *w, const uint8_t *sg_in,
+                               unsigned long long S,
+                               unsigned long long *rec, uint8_t *sg_out) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  uint32_t v = vals[i];
+  rec[i] = (unsigned long long)tp[v] |
+           ((unsigned long long)__float_as_uint(w[v]) << 32);
+  if (sg_out) sg_out[i] = sg_in[v];
+}
+
+__global__ void segments(const unsigned long long *off, uint32_t N, uint32_t D,
+                         unsigned long long *base, uint32_t *seg) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > N) return;
+  unsigned long long b = off[(unsigned long long)s * D];
+  base[s] = b;
+  if (s == N) return;
+  for (uint32_t d = 0; d <= D; d++)
+    seg[(unsigned long long)s * (D + 1) + d] =
+        (uint32_t)(off[(unsigned long long)s * D + d] - b);
+}
+
+// reverse index of plastic synapses: key = target, stable by synapse index
+__global__ void rev_keys(const unsigned long long *rec, const uint8_t *sg,
+                         unsigned long long S, uint32_t *keys, uint32_t *vals,
+                         uint32_t *count, uint32_t *n_out) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  if (sg[i] == 0) return;
+  uint32_t t = (uint32_t)rec[i] & kTargetMask;
+  uint32_t pos = atomicAdd(n_out, 1u);
+  keys[pos] = t;
+  vals[pos] = (uint32_t)i;
+  atomicAdd(count + t, 1u);
+}
+
+template <class T>
+struct DTmp {
+  T *p = nullptr;
+  explicit DTmp(size_t n) {
+    if (hipMalloc((void **)&p, (n ? n : 1) * sizeof(T)) != hipSuccess) p = nullptr;
+  }
+  ~DTmp() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int bits_for(unsigned long long v) {
+  int b = 0;
+  while (b < 32 && (1ull << b) <= v) b++;
+  return b < 1 ? 1 : b;
+}
+
+}  // namespace
+
+const char *BuildSynapseStore(const uint32_t *src, const uint32_t *tp,
+                              const float *w, const uint16_t *d,
+                              const uint8_t *sg_in, unsigned long long S,
+                              uint32_t N, uint32_t D,
+                              unsigned long long *rec, uint8_t *sg_out,
+                              unsigned long long *base, uint32_t *seg,
+                              void *stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  unsigned long long nk = (unsigned long long)N * D;
+  if (nk >= 0xFFFFFFFFull) return "source x delay key space exceeds 32 bits";
+  DTmp<uint32_t> keys(S), keys2(S), vals(S), vals2(S), count(nk + 1);
+  DTmp<unsigned long long> off(nk + 1);
+  if (!keys.p || !keys2.p || !vals.p || !vals2.p || !count.p || !off.p)
+    return "out of device memory building the synapse store";
+  if (hipMemsetAsync(count.p, 0, (nk + 1) * sizeof(uint32_t), stream) != hipSuccess)
+    return "hipMemsetAsync failed";
+  unsigned int grid = (unsigned int)((S + 255) / 256);
+  if (S > 0)
+    hipLaunchKernelGGL(make_keys, dim3(grid), dim3(256), 0, stream, src, d, D,
+                       S, keys.p, vals.p, count.p);
+  // exclusive scan of the per-key counts -> first position of each key
+  size_t tmp_bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, count.p, off.p,
+                                   (int)(nk + 1), stream);
+  size_t sort_bytes = 0;
+  hipcub::DoubleBuffer<uint32_t> kb(keys.p, keys2.p), vb(vals.p, vals2.p);
+  int end_bit = bits_for(nk);
+  if (S > 0)
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, kb, vb, (int)S, 0,
+                                       end_bit, stream);
+  DTmp<unsigned char> tmp(tmp_bytes > sort_bytes ? tmp_bytes : sort_bytes);
+  if (!tmp.p) return "out of device memory building the synapse store";
+  if (hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, count.p, off.p,
+                                       (int)(nk + 1), stream) != hipSuccess)
+    return "scan failed";
+  if (S > 0) {
+    size_t sb = tmp_bytes > sort_bytes ? tmp_bytes : sort_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, sb, kb, vb, (int)S, 0,
+                                           end_bit, stream) != hipSuccess)
+      return "radix sort failed";
+    hipLaunchKernelGGL(gather_records, dim3(grid), dim3(256), 0, stream,
+                       vb.Current(), tp, w, sg_in, S, rec, sg_out);
+  }
+  hipLaunchKernelGGL(segments, dim3((N + 1 + 255) / 256), dim3(256), 0, stream,
+                     off.p, N, D, base, seg);
+  if (hipStreamSynchronize(stream) != hipSuccess) return "synapse store build failed";
+  return nullptr;
+}
+
+const char *BuildReverseIndex(const unsigned long long *rec,
+                              const uint8_t *sg, unsigned long long S,
+                              uint32_t N, unsigned long long *rev_ptr,
+                              uint32_t **rev_idx_out,
+                              unsigned long long *n_rev_out, void *stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  DTmp<uint32_t> keys(S), keys2(S), vals(S), vals2(S), count(N + 1), n_out(1);
+  if (!keys.p || !keys2.p || !vals.p || !vals2.p || !count.p || !n_out.p)
+    return "out of device memory building the reverse index";
+  (void)hipMemsetAsync(count.p, 0, (N + 1) * sizeof(uint32_t), stream);
+  (void)hipMemsetAsync(n_out.p, 0, sizeof(uint32_t), stream);
+  unsigned int grid = (unsigned int)((S + 255) / 256);
+  if (S > 0)
+    hipLaunchKernelGGL(rev_keys, dim3(grid), dim3(256), 0, stream, rec, sg, S,
+                       keys.p, vals.p, count.p, n_out.p);
+  uint32_t n_rev = 0;
+  (void)hipMemcpyAsync(&n_rev, n_out.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       stream);
+  if (hipStreamSynchronize(stream) != hipSuccess) return "reverse index build failed";
+  // the atomic append order is arbitrary: sort by (target, synapse index)
+  // in two stable passes (index first, then target) for canonical order
+  size_t tmp_bytes = 0, b2 = 0;
+  hipcub::DoubleBuffer<uint32_t> kb(keys.p, keys2.p), vb(vals.p, vals2.p);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, vb, kb, (int)n_rev, 0,
+                                     32, stream);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b2, count.p, rev_ptr, (int)(N + 1),
+                                   stream);
+  size_t tb = tmp_bytes > b2 ? tmp_bytes : b2;
+  DTmp<unsigned char> tmp(tb);
+  if (!tmp.p) return "out of device memory building the reverse index";
+  if (n_rev > 0) {
+    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, vb, kb, (int)n_rev, 0, 32,
+                                           stream) != hipSuccess)
+      return "radix sort failed";
+    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, kb, vb, (int)n_rev, 0,
+                                           bits_for(N), stream) != hipSuccess)
+      return "radix sort failed";
+  }
+  if (hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, count.p, rev_ptr,
+                                       (int)(N + 1), stream) != hipSuccess)
+    return "scan failed";
+  uint32_t *idx = nullptr;
+  if (hipMalloc((void **)&idx, ((size_t)n_rev + 1) * sizeof(uint32_t)) != hipSuccess)
+    return "out of device memory building the reverse index";
+  if (n_rev > 0)
+    (void)hipMemcpyAsync(idx, vb.Current(), (size_t)n_rev * sizeof(uint32_t),
+                         hipMemcpyDeviceToDevice, stream);
+  if (hipStreamSynchronize(stream) != hipSuccess) return "reverse index build failed";
+  *rev_idx_out = idx;
+  *n_rev_out = n_rev;
+  return nullptr;
+}
+
+}  // namespace ngpu

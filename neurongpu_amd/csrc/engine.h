@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -147,11 +148,18 @@ class Engine {
   std::vector<int> node_group;
   int n_node = 0;
 
-  // connections, coordinate form, kept until calibration
-  std::vector<uint32_t> c_src, c_tp;  // tp = target | port << 26
-  std::vector<float> c_w;
-  std::vector<uint16_t> c_d;          // delay index d_int
-  std::vector<uint8_t> c_sg;
+  // connections in coordinate form and insertion order, one chunk per
+  // Connect call, kept until calibration
+  struct CooChunk {
+    size_t n = 0;
+    std::unique_ptr<uint32_t[]> src, tp;  // tp = target | port << 26
+    std::unique_ptr<float[]> w;
+    std::unique_ptr<uint16_t[]> d;        // delay index d_int
+    std::unique_ptr<uint8_t[]> sg;
+  };
+  std::vector<CooChunk> coo;
+  size_t n_conn = 0;
+  int coo_D = 0;                      // max delay index + 1
   bool any_plastic = false;
 
   // canonical CSR on the host (built at calibration, or on demand)
@@ -162,6 +170,7 @@ class Engine {
   std::vector<unsigned long long> h_rec;
   std::vector<uint8_t> h_sg;
   bool h_rec_stale = false;           // device weights changed (plasticity)
+  size_t n_syn = 0;                   // synapses in the device store
 
   std::vector<SynGroupDesc> syn_groups;
 
@@ -207,6 +216,17 @@ class Engine {
   long long rec_rows_cap = 0, rec_rows = 0;
   std::vector<double> rec_times;
 
+  // optional per-kernel timing with HIP events on the engine stream
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_kind;
+  size_t ev_used = 0;
+  double kernel_ms[8] = {0};
+  long long kernel_cnt[8] = {0};
+  void TimeBegin(int kind);
+  void TimeEnd();
+  void CollectTimes();
+
   // helpers
   int GroupOf(int i_node, int n_node_req) const;
   int GroupOfList(const int *nodes, int n) const;
@@ -223,6 +243,10 @@ class Engine {
   void LaunchGroup(Group &g, bool do_step, long long T, double t,
                    long long it_local);
   void EnsureHostCSR();
+  void BuildDeviceStore();
+  // canonical rows of one source: delay segment offsets and records
+  void SourceRows(int s, std::vector<uint32_t> &seg,
+                  std::vector<unsigned long long> &rec);
   FoldIn MakeFold(long long pois_T, double pois_t);
   SpikeOut MakeSpikeOut(Group &g, long long T, double t);
 };

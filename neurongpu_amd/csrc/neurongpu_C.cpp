@@ -448,3 +448,10 @@ int NeuronGPU_GetHotPathStats(unsigned long long *out3)
     })
 
 }  // extern "C"
+
+extern "C" {
+int NeuronGPU_SetKernelTiming(int on)
+    RET(int, 0, ret = instance->SetKernelTiming(on))
+int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8)
+    RET(int, 0, ret = instance->GetKernelTimes(ms8, cnt8))
+}

@@ -185,6 +185,9 @@ _SIG = {
     "RemoteCreate": (c_int, (c_int, c_char_p, c_int, c_int)),
     "ResetKernel": (c_int, ()),
     "GetHotPathStats": (c_int, (ctypes.POINTER(ctypes.c_ulonglong),)),
+    "SetKernelTiming": (c_int, (c_int,)),
+    "GetKernelTimes": (c_int, (ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_longlong))),
 }
 
 _C = {}
@@ -1345,3 +1348,20 @@ def GetHotPathStats():
     a = (ctypes.c_ulonglong * 3)()
     _call("GetHotPathStats", a)
     return {"rk5_accepted": a[0], "rk5_rejected": a[1], "syn_events": a[2]}
+
+
+KERNEL_KINDS = ("aeif_rk5", "iaf_psc_exp", "relay", "spike_generator",
+                "delivery", "stdp_post", "record", "unused")
+
+
+def SetKernelTiming(on):
+    "Enable per-kernel HIP-event timing (clears the totals)"
+    return _call("SetKernelTiming", 1 if on else 0)
+
+
+def GetKernelTimes():
+    "{kind: (total_ms, launches)} of the timed kernels"
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_longlong * 8)()
+    _call("GetKernelTimes", ms, cnt)
+    return {KERNEL_KINDS[k]: (ms[k], cnt[k]) for k in range(8) if cnt[k]}

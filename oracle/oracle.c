@@ -721,8 +721,9 @@ int orc_net_calibrate(orc_net *net) {
   long nc = net->n_conn;
   for (long k = 0; k < nc; k++)
     if (net->conn[k].delay < net->dt) return -1; /* src/connect.cu:29 */
-  qsort(net->conn, nc, sizeof(orc_conn), conn_cmp);
-  net->src_ptr = (long *)calloc(net->n_node + 1, sizeof(long));
+  /* canonical order (source, delay index, insertion): stable counting
+   * sort by source, then stable sort by delay inside each source */
+  net->src_ptr = (long *)calloc((size_t)net->n_node + 1, sizeof(long));
   net->max_d = 0;
   for (long k = 0; k < nc; k++) {
     net->src_ptr[net->conn[k].src + 1]++;
@@ -730,6 +731,23 @@ int orc_net_calibrate(orc_net *net) {
     if (net->conn[k].syn_group > 0) net->any_plastic = 1;
   }
   for (int i = 0; i < net->n_node; i++) net->src_ptr[i + 1] += net->src_ptr[i];
+  {
+    orc_conn *tmp = (orc_conn *)malloc((nc + 1) * sizeof(orc_conn));
+    long *fill = (long *)malloc(((size_t)net->n_node + 1) * sizeof(long));
+    memcpy(fill, net->src_ptr, (size_t)net->n_node * sizeof(long));
+    for (long k = 0; k < nc; k++) tmp[fill[net->conn[k].src]++] = net->conn[k];
+    free(fill);
+    free(net->conn);
+    net->conn = tmp;
+    net->cap_conn = nc + 1;
+    for (int s = 0; s < net->n_node; s++) {
+      long b = net->src_ptr[s], e = net->src_ptr[s + 1];
+      int sorted = 1;
+      for (long k = b + 1; k < e; k++)
+        if (net->conn[k].d < net->conn[k - 1].d) { sorted = 0; break; }
+      if (!sorted) qsort(net->conn + b, e - b, sizeof(orc_conn), conn_cmp);
+    }
+  }
   /* reverse plastic index, src/spike_buffer.cu:423-470 */
   net->rev_ptr = (long *)calloc(net->n_node + 1, sizeof(long));
   long n_rev = 0;

@@ -275,8 +275,15 @@ def main():
     ap.add_argument("--neurons", type=int, default=1000000)
     ap.add_argument("--timing-steps", type=int, default=300)
     ap.add_argument("--no-readme", action="store_true")
+    ap.add_argument("--only-readme", action="store_true",
+                    help="measure only the README network (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-neurons", type=int, default=100000)
+    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (MiB) "
+                         "with torch, to scale rocprofv3 FETCH_SIZE/"
+                         "WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU "
@@ -285,6 +292,12 @@ def main():
 
     from neurongpu_amd import neurongpu as ngpu
 
+    if args.only_readme:
+        neuron, n_syn = build_readme(ngpu, args.neurons)
+        rd = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
+                     args.timing_steps)
+        print(json.dumps({"readme_network": rd}), flush=True)
+        return
     log("building c1 (%d neurons)" % args.neurons)
     neuron, n_syn = build_c1(ngpu, args.neurons)
     c1 = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
@@ -344,7 +357,16 @@ def main():
         ngpu.ResetKernel()
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_neurons)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_neurons, args.cpu_steps)
+    if args.copy_calib_mib > 0:
+        import torch
+        n = args.copy_calib_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 4 * n
     print(json.dumps(out), flush=True)
 
 

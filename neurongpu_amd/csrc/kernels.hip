@@ -645,6 +645,23 @@ __global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs A) {
   }
 }
 
+// Spikes of nodes with a dendritic delay of k > 0 steps reach the
+// dendrites k steps after emission (src/spike_buffer.cu:276-278,299-301):
+// mark them before this step's delivery reads last_rev.
+__global__ __launch_bounds__(kBlock) void den_rev_kernel(RevArgs A) {
+  for (int k = 1; k <= A.max_den; k++) {
+    long long s = A.gs - k;
+    if (s < 0) break;
+    long long slot = s % A.R;
+    uint32_t n = A.ring_n[slot];
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < n;
+         e += gridDim.x * kBlock) {
+      uint32_t node = A.ring_src[slot * A.cap + e];
+      if (A.den_idx[node] == k) A.last_rev[node] = A.T;
+    }
+  }
+}
+
 __global__ void record_kernel(RecordArgs A) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= A.n_col) return;
@@ -729,6 +746,11 @@ void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream) {
 
 void LaunchRevStdp(const RevArgs &a, int grid, void *stream) {
   hipLaunchKernelGGL(rev_kernel, dim3(grid), dim3(kBlock), 0,
+                     (hipStream_t)stream, a);
+}
+
+void LaunchDenRev(const RevArgs &a, int grid, void *stream) {
+  hipLaunchKernelGGL(den_rev_kernel, dim3(grid), dim3(kBlock), 0,
                      (hipStream_t)stream, a);
 }
 

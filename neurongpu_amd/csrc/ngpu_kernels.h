@@ -172,7 +172,7 @@ struct RevArgs {
   const uint16_t *spike_time;
   const float *stdp_param;
   const int *syn_type;
-  const long long *last_rev;
+  long long *last_rev;                // written by the dendritic-delay pass
   long long T;
   float dt;
 };
@@ -194,6 +194,7 @@ void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
 void LaunchRevStdp(const RevArgs &a, int grid, void *stream);
+void LaunchDenRev(const RevArgs &a, int grid, void *stream);
 void LaunchRecord(const RecordArgs &a, void *stream);
 void LaunchSpikeTimeWrap(uint16_t *spike_time, unsigned long long n, int up,
                          void *stream);

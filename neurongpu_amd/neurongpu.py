@@ -206,6 +206,19 @@ def _b(s):
     raise ValueError("Variable cannot be converted to string")
 
 
+to_byte_str = _b  # pythonlib/neurongpu.py:74-80
+
+
+def to_def_str(s):
+    "bytes -> str (pythonlib/neurongpu.py:82-86)"
+    return s.decode("utf-8")
+
+
+def waitenter(val):
+    "Prompt and wait for Enter (pythonlib/neurongpu.py:88-92)"
+    return input(val)
+
+
 def GetErrorMessage():
     "Get error message from NeuronGPU exception"
     m = _C["GetErrorMessage"]()
@@ -951,9 +964,14 @@ def SetSynSpecFloatParam(param_name, val):
 
 
 def SetSynSpecFloatPtParam(param_name, arr):
+    """Pointer parameter (weight_array, delay_array); the caller keeps the
+    array alive until Connect has returned (pythonlib/neurongpu.py:1424-1433).
+    """
+    if isinstance(arr, (list, tuple)):
+        arr = (ctypes.c_float * len(arr))(*arr)
     if isinstance(arr, np.ndarray):
         arr = arr.ctypes.data_as(c_void_p)
-    return _call("SetSynSpecFloatPtParam", _b(param_name), arr)
+    return _call("SetSynSpecFloatPtParam", _b(param_name), ctypes.cast(arr, c_void_p))
 
 
 def SynSpecIsIntParam(param_name):
@@ -966,6 +984,19 @@ def SynSpecIsFloatParam(param_name):
 
 def SynSpecIsFloatPtParam(param_name):
     return _call("SynSpecIsFloatPtParam", _b(param_name)) != 0
+
+
+def SetSynParamFromArray(param_name, par_dict, array_size):
+    """Set <param_name>_array of the synapse spec from an array or
+    distribution dictionary (pythonlib/neurongpu.py:1533-1541).  The array
+    is returned so the caller can keep it alive until Connect."""
+    arr_name = param_name + "_array"
+    if not SynSpecIsFloatPtParam(arr_name):
+        raise ValueError("Synapse parameter cannot be set by"
+                         " arrays or distributions")
+    arr = DictToArray(par_dict, array_size)
+    SetSynSpecFloatPtParam(arr_name, arr)
+    return arr
 
 
 def RuleArraySize(conn_dict, source, target):
@@ -1012,9 +1043,11 @@ def _connect_specs(source, target, conn_dict, syn_dict, keep):
             else:
                 SetSynSpecFloatParam(k, v)
         elif SynSpecIsFloatPtParam(k):
-            if isinstance(v, np.ndarray):
+            if isinstance(v, (list, tuple)):
+                v = (ctypes.c_float * len(v))(*v)
+            elif isinstance(v, np.ndarray):
                 v = np.ascontiguousarray(v, dtype=np.float32)
-                keep.append(v)
+            keep.append(v)
             SetSynSpecFloatPtParam(k, v)
         else:
             raise ValueError("Unknown synapse parameter")

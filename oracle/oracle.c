@@ -530,7 +530,7 @@ static void model_sizes(int model, int n_port, int *nvar, int *nparam) {
   case ORC_AEIF_COND_ALPHA: *nvar = 2 + 2 * n_port; *nparam = 14 + 3 * n_port; break;
   case ORC_IAF_PSC_EXP: *nvar = 4; *nparam = 16; break;
   case ORC_POISSON_GENERATOR: *nvar = 0; *nparam = 4; break;
-  case ORC_PARROT_NEURON: *nvar = 1; *nparam = 1; break;
+  case ORC_PARROT_NEURON: *nvar = 3; *nparam = 2; break;
   default: *nvar = 0; *nparam = 0; break;
   }
 }
@@ -542,7 +542,7 @@ int orc_net_add_group(orc_net *net, int model, int n, int n_port) {
   g->n = n;
   if (model == ORC_IAF_PSC_EXP) n_port = 2;
   if (model == ORC_SPIKE_GENERATOR || model == ORC_POISSON_GENERATOR) n_port = 0;
-  if (model == ORC_PARROT_NEURON) n_port = 1;
+  if (model == ORC_PARROT_NEURON) n_port = 2;
   g->n_port = n_port;
   g->i0 = net->n_node;
   model_sizes(model, n_port, &g->nvar, &g->nparam);
@@ -826,6 +826,7 @@ int orc_net_calibrate(orc_net *net) {
     int den_p = -1;
     if (g->model == ORC_AEIF_COND_BETA || g->model == ORC_AEIF_COND_ALPHA) den_p = 13;
     if (g->model == ORC_IAF_PSC_EXP) den_p = 9;
+    if (g->model == ORC_PARROT_NEURON) den_p = 1;
     if (den_p >= 0)
       for (int i = 0; i < g->n; i++)
         net->den_idx[g->i0 + i] = (int)ROUNDR(*P(g, den_p, i) / (REAL)net->dt);
@@ -1031,7 +1032,17 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
         }
       }
     } else if (g->model == ORC_PARROT_NEURON) {
-      /* not modelled */
+      /* src/parrot_neuron.cu:48-70 */
+      for (int i = 0; i < g->n; i++) {
+        REAL *in = V(g, 0, i);
+        REAL sh = *in;
+        *V(g, 2, i) = sh;
+        if (sh != (REAL)0) {
+          if (*P(g, 0, i) == (REAL)0) sh = (REAL)1;
+          *in = (REAL)0;
+          push_spike(net, g->i0 + i, sh, T, t);
+        }
+      }
     }
   }
   record_row(net, t);
@@ -1108,6 +1119,12 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
           REAL *I = V(g, q, i);
           *I = *I + acc[(size_t)q * nn + g->i0 + i];
         }
+    } else if (g->model == ORC_PARROT_NEURON) {
+      /* port 0 weight 1 into input_spike_height; port 1 weight 0 */
+      for (int i = 0; i < g->n; i++) {
+        REAL *I = V(g, 0, i);
+        *I = *I + acc[g->i0 + i];
+      }
     }
   }
   memset(acc, 0, (size_t)(net->max_port > 0 ? net->max_port : 1) * nn * sizeof(REAL));

@@ -21,6 +21,7 @@ AEIF_COND_ALPHA = 2
 IAF_PSC_EXP = 3
 SPIKE_GENERATOR = 4
 POISSON_GENERATOR = 5
+PARROT_NEURON = 6
 
 MODEL_IDS = {
     "aeif_cond_beta": AEIF_COND_BETA,
@@ -28,6 +29,7 @@ MODEL_IDS = {
     "iaf_psc_exp": IAF_PSC_EXP,
     "spike_generator": SPIKE_GENERATOR,
     "poisson_generator": POISSON_GENERATOR,
+    "parrot_neuron": PARROT_NEURON,
 }
 
 AEIF_SCAL_PARAM = ["V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w",
@@ -144,6 +146,8 @@ class Group:
             return AEIF_SCAL_VAR, AEIF_PORT_VAR
         if self.model == "iaf_psc_exp":
             return IAF_VAR, []
+        if self.model == "parrot_neuron":
+            return ["input_spike_height", "dummy_input", "V"], []
         return [], []
 
     def param_names(self):
@@ -153,6 +157,8 @@ class Group:
             return AEIF_SCAL_PARAM, ALPHA_PORT_PARAM
         if self.model == "iaf_psc_exp":
             return IAF_PARAM, []
+        if self.model == "parrot_neuron":
+            return ["hold_spike_height", "den_delay"], []
         return [], []
 
     @staticmethod
@@ -236,7 +242,7 @@ class Net:
     def create(self, model, n=1, n_port=1):
         gid = len(self.groups)
         i0 = self.L.orc_net_add_group(self.h, MODEL_IDS[model], n, n_port)
-        if model == "iaf_psc_exp":
+        if model in ("iaf_psc_exp", "parrot_neuron"):
             n_port = 2
         if model in ("spike_generator", "poisson_generator"):
             n_port = 0

@@ -214,3 +214,138 @@ def brunel_spec(order=200, seed=1234, w_ex=0.05, w_in=0.35, w_p=0.37,
                        "weight": np.full(n, w_p, np.float32),
                        "delay": np.full(n, 0.2, np.float32), "port": 0}],
             "poisson": [{"node": n, "rate": rate}]}
+
+
+def load_stdp_cases():
+    import json
+    import os
+    p = os.path.join(os.path.dirname(__file__), "golden", "stdp_cases.json")
+    with open(p) as f:
+        return json.load(f)
+
+
+def stdp_case_spec(case, common):
+    """Network of python/test/test_stdp/cases/case<k>.py: two parrot
+    neurons driven by spike generators (weight 1, delay 1 ms), an STDP
+    synapse pre -> post on receptor 1 (port weight 0, so the post neuron
+    never relays the pre spikes), dendritic delay on the post neuron."""
+    stdp = {k: common[k] for k in ("tau_plus", "tau_minus", "lambda", "alpha",
+                                   "mu_plus", "mu_minus", "Wmax")}
+    return {
+        "dt": 0.1,
+        "groups": [
+            {"model": "parrot_neuron", "n": 1},                    # 0 pre
+            {"model": "parrot_neuron", "n": 1,
+             "params": {"den_delay": case["den_delay"]}},          # 1 post
+            {"model": "spike_generator", "n": 1},                  # 2
+            {"model": "spike_generator", "n": 1},                  # 3
+        ],
+        "spike_gen": [{"node": 2, "times": case["pre"]},
+                      {"node": 3, "times": case["post"]}],
+        "stdp": [stdp],
+        "conns": [
+            {"src": [2], "tgt": [0], "weight": 1.0,
+             "delay": common["gen_delay"]},
+            {"src": [3], "tgt": [1], "weight": 1.0,
+             "delay": common["gen_delay"]},
+            {"src": [0], "tgt": [1], "weight": common["weight"],
+             "delay": case["delay"], "port": common["receptor"],
+             "syn_group": 1},
+        ],
+    }
+
+
+def stdp_closed_form(w, dts, common):
+    """Weight after the sequence of updates of the reference's test
+    (STDPUpdate of python/test/test_stdp/cases/case1.py:7-21)."""
+    import math
+    wplus = common["Wmax"] * common["lambda"]
+    for dt in dts:
+        if dt >= 0:
+            fact = wplus * math.exp(-dt / common["tau_plus"])
+            w = w + fact * math.pow(1.0 - w / common["Wmax"], common["mu_plus"])
+            w = min(w, common["Wmax"])
+        else:
+            fact = -common["alpha"] * wplus * math.exp(dt / common["tau_minus"])
+            w = w + fact * math.pow(w / common["Wmax"], common["mu_minus"])
+            w = max(w, 0.0)
+    return w
+
+
+def nest_spec(model):
+    """Single neuron with 3 ports driven by a spike generator at 10 and
+    400 ms, delays 1/100/130 ms (python/test/test_aeif_cond_beta.py:4-30)."""
+    if model == "iaf_psc_exp":
+        return {"groups": [{"model": "iaf_psc_exp", "n": 1},
+                           {"model": "spike_generator", "n": 1}],
+                "spike_gen": [{"node": 1, "times": [10.0, 400.0]}],
+                "conns": [{"src": [1], "tgt": [0], "weight": [1.0],
+                           "delay": [1.0], "port": 0},
+                          {"src": [1], "tgt": [0], "weight": [-2.0],
+                           "delay": [100.0], "port": 1}],
+                "records": [(0, "V_m_rel", 0)]}
+    params = {"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6, "g_L": 300.0,
+              "E_rev": [20.0, 0.0, -85.0]}
+    if model == "aeif_cond_beta":
+        params.update({"tau_decay": [40.0, 20.0, 30.0],
+                       "tau_rise": [20.0, 10.0, 5.0]})
+    else:
+        params.update({"tau_syn": [40.0, 20.0, 30.0]})
+    spec = {"groups": [{"model": model, "n": 1, "n_port": 3,
+                        "params": params},
+                       {"model": "spike_generator", "n": 1}],
+            "spike_gen": [{"node": 1, "times": [10.0, 400.0]}],
+            "conns": [], "records": [(0, "V_m", 0)]}
+    for syn, (w, d) in enumerate(zip([0.1, 0.2, 0.5], [1.0, 100.0, 130.0])):
+        spec["conns"].append({"src": [1], "tgt": [0], "weight": [w],
+                              "delay": [d], "port": syn})
+    return spec
+
+
+def stdp_net_spec(n_src=200, n_post=100, k_plastic=20, seed=99,
+                  den_delay=1.5, max_delay=5.0):
+    """Plastic network whose plastic weights cannot feed back into the
+    dynamics, so spike trains and final weights are comparable bit for bit
+    with any accumulation order: aeif_cond_beta sources driven by constant
+    current plus their own Poisson generator (weight 0.25 nS), parrot
+    neurons relaying 3 sources each on port 0 (weight 1), and STDP synapses
+    (fixed indegree k_plastic, random weights and delays) from the sources
+    onto port 1 of the parrots, whose input weight is 0.  The parrots carry
+    a dendritic delay."""
+    rng = np.random.default_rng(seed)
+    I_e = rng.uniform(450.0, 900.0, n_src).astype(np.float32)
+    n_gen = n_src
+    gen0 = n_src + n_post
+    src_p = np.concatenate([rng.choice(n_src, k_plastic, replace=False)
+                            for _ in range(n_post)])
+    tgt_p = n_src + np.repeat(np.arange(n_post), k_plastic)
+    w_p = rng.uniform(0.5, 5.0, src_p.size).astype(np.float32)
+    d_p = (rng.integers(1, int(max_delay / 0.1) + 1, src_p.size)
+           * 0.1).astype(np.float32)
+    drv_src = np.concatenate([rng.choice(n_src, 3, replace=False)
+                              for _ in range(n_post)])
+    drv_tgt = n_src + np.repeat(np.arange(n_post), 3)
+    return {
+        "dt": 0.1, "seed": seed,
+        "groups": [
+            {"model": "aeif_cond_beta", "n": n_src, "n_port": 2,
+             "params": {"I_e": I_e, "E_rev": [0.0, -85.0],
+                        "tau_rise": [0.2, 0.5], "tau_decay": [2.0, 10.0]},
+             "rk5": {"rk5_tol": 1e-3}},
+            {"model": "parrot_neuron", "n": n_post,
+             "params": {"den_delay": den_delay}},
+            {"model": "poisson_generator", "n": n_gen},
+        ],
+        "poisson": [{"node": gen0 + i, "rate": 3000.0} for i in range(n_gen)],
+        "stdp": [{"tau_plus": 20.0, "tau_minus": 20.0, "lambda": 0.01,
+                  "alpha": 1.05, "mu_plus": 1.0, "mu_minus": 1.0,
+                  "Wmax": 10.0}],
+        "conns": [
+            {"src": gen0 + np.arange(n_gen), "tgt": np.arange(n_src),
+             "weight": 0.25, "delay": 0.1, "port": 0},
+            {"src": drv_src, "tgt": drv_tgt, "weight": 1.0, "delay": 0.5,
+             "port": 0},
+            {"src": src_p, "tgt": tgt_p, "weight": w_p, "delay": d_p,
+             "port": 1, "syn_group": 1},
+        ],
+    }

@@ -1,0 +1,332 @@
+"""Host-side API behaviour through the C ABI, no GPU needed: node creation,
+parameter and variable access, connection rules, connection queries,
+synapse groups and error handling happen on the host before calibration.
+
+The scenario tests repeat the calls of the reference's own scripts
+(python/test/test_connect.py, test_getarr.py, test_setvar2.py,
+test_syn_group.py) and compare what they print with the reference's
+golden logs (python/test/logp3_*.txt, banner removed), kept under
+tests/golden/."""
+import contextlib
+import io
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def golden_lines(name):
+    with open(os.path.join(HERE, "golden", name)) as f:
+        return f.read().rstrip("\n").split("\n")
+
+
+def printed(fn, ng):
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        fn(ng)
+    return buf.getvalue().rstrip("\n").split("\n")
+
+
+def strip_trailing_blank(lines):
+    while lines and lines[-1].strip() == "":
+        lines = lines[:-1]
+    return lines
+
+
+# python/test/test_connect.py:1-94
+def scenario_connect(ng):
+    N = 5
+    neuron = ng.Create("aeif_cond_beta", 2 * N)
+    even = [neuron[2 * i] for i in range(N)]
+    odd = [neuron[2 * i + 1] for i in range(N)]
+    e2o_d, e2o_w, o2e_d, o2e_w = [], [], [], []
+    for it in range(N):
+        ite, ito = 2 * it, 2 * it + 1
+        for isrc in range(N):
+            ise, iso = 2 * isrc, 2 * isrc + 1
+            e2o_d.append(2.0 * N * ise + ito)
+            e2o_w.append(100.0 * (2.0 * N * ise + ito))
+            o2e_d.append(2.0 * N * iso + ite)
+            o2e_w.append(100.0 * (2.0 * N * iso + ite))
+    conn = {"rule": "all_to_all"}
+    ng.Connect(even, odd, conn, {"weight_array": e2o_w, "delay_array": e2o_d})
+    ng.Connect(odd, even, conn, {"weight_array": o2e_w, "delay_array": o2e_d})
+
+    def dump(title, src, tgt, keys=None):
+        cid = ng.GetConnections(src, tgt)
+        st = ng.GetStatus(cid) if keys is None else ng.GetStatus(cid, keys)
+        print("########################################")
+        print(title)
+        for s in st:
+            print(s)
+        print()
+        print()
+
+    dump("Even to all", even, neuron)
+    dump("Even to all weight, delat", even, neuron, ["weight", "delay"])
+    dump("All to odd", neuron, odd)
+    n36 = neuron[3:7]
+    dump("Even to 3,4,5,6", even, n36)
+    dump("3,4,5,6 to odd", n36, odd)
+
+
+# python/test/test_getarr.py:1-50
+def scenario_getarr(ng):
+    spike = ng.Create("spike_generator", 4)
+    s0, s1, s2 = spike[0:2], spike[2:3], spike[3:4]
+    ng.SetStatus(s0, {"spike_times": [10.0, 400.0], "spike_heights": [1.0, 0.5]})
+    ng.SetStatus(s1, {"spike_times": [4.0], "spike_heights": [2.0]})
+    ng.SetStatus(s2, {"spike_times": [50.0, 20.0, 80.0],
+                      "spike_heights": [0.1, 0.3, 0.2]})
+    for s in (s0, s1, s2):
+        print(ng.GetStatus(s, "spike_times"))
+        print(ng.GetStatus(s, "spike_heights"))
+        if s is not s2:
+            print()
+    print()
+    print()
+    nl = [spike[2], spike[3], spike[0], spike[1]]
+    print(ng.GetStatus(nl, "spike_times"))
+    print(ng.GetStatus(nl, "spike_heights"))
+    print()
+    print()
+    print(ng.GetStatus(s1))
+    print()
+    print()
+    print(ng.GetStatus(nl))
+
+
+# python/test/test_setvar2.py:1-52
+def scenario_setvar2(ng):
+    neuron = ng.Create("aeif_cond_beta", 6, 2)
+    even = [neuron[0], neuron[2], neuron[4]]
+    odd = [neuron[3], neuron[5], neuron[1]]
+    ng.SetStatus(even, {"tau_decay": [80.0, 40.0], "tau_rise": [60.0, 20.0]})
+    ng.SetStatus(odd, {"tau_decay": [70.0, 30.0], "tau_rise": [50.0, 10.0]})
+    ng.SetStatus(even, {"V_m": -80.0})
+    ng.SetStatus(odd, {"V_m": -90.0})
+    ng.SetStatus(even, {"g1": [0.4, 0.2]})
+    ng.SetStatus(odd, {"g1": [0.3, 0.1]})
+    ng.SetStatus(even, {"V_th": -40.0})
+    ng.SetStatus(odd, {"V_th": -30.0})
+    names = ["tau_decay", "tau_rise", "V_m", "V_th", "g1"]
+    tags = ["td", "tr", "Vm", "Vth", "g1"]
+    for tag, nm in zip(tags, names):
+        print("read_" + tag, ng.GetNeuronStatus(neuron, nm))
+    nl = [neuron[0], neuron[2], neuron[4], neuron[1], neuron[3], neuron[5]]
+    for tag, nm in zip(tags, names):
+        print("read1_" + tag, ng.GetNeuronStatus(nl, nm))
+
+
+# python/test/test_syn_group.py:1-30
+def scenario_syn_group(ng):
+    sg = ng.CreateSynGroup("test_syn_model", {"fact": 1.0, "offset": 2.0})
+    print(ng.GetStatus(sg))
+    ng.SetStatus(sg, "fact", 3.0)
+    ng.SetStatus(sg, "offset", 4.0)
+    print(ng.GetStatus(sg))
+    ng.SetStatus(sg, {"fact": 5.0, "offset": 6.0})
+    print(ng.GetStatus(sg))
+    print("fact: ", ng.GetSynGroupParam(sg, "fact"))
+    print("offset: ", ng.GetSynGroupParam(sg, "offset"))
+    print(ng.GetStatus(sg, "fact"))
+    print(ng.GetStatus(sg, "offset"))
+    print(ng.GetStatus(sg, ["fact", "offset"]))
+    print(ng.GetStatus(sg, ["offset", "fact"]))
+
+
+@pytest.mark.parametrize("name,fn", [
+    ("connect", scenario_connect),
+    ("getarr", scenario_getarr),
+    ("setvar2", scenario_setvar2),
+    ("syn_group", scenario_syn_group),
+])
+def test_reference_log(ngpu, name, fn):
+    got = strip_trailing_blank(printed(fn, ngpu))
+    want = strip_trailing_blank(golden_lines("logp3_%s.txt" % name))
+    assert got == want
+
+
+# ---------------------------------------------------------------------
+# connection rules (src/connect_rules.h; counts follow the rule text)
+# ---------------------------------------------------------------------
+def _conns(ng, src, tgt):
+    st = ng.GetStatus(ng.GetConnections(src, tgt))
+    return [(d["source"], d["target"]) for d in st]
+
+
+def test_rule_one_to_one(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 7)
+    b = ngpu.Create("aeif_cond_beta", 7)
+    ngpu.Connect(a, b, {"rule": "one_to_one"}, {"weight": 1.0, "delay": 1.0})
+    assert _conns(ngpu, a, b) == [(a[i], b[i]) for i in range(7)]
+
+
+def test_rule_all_to_all(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 4)
+    b = ngpu.Create("aeif_cond_beta", 3)
+    ngpu.Connect(a, b, {"rule": "all_to_all"}, {"weight": 1.0, "delay": 1.0})
+    got = sorted(_conns(ngpu, a, b))
+    assert got == sorted((a[i], b[j]) for i in range(4) for j in range(3))
+
+
+def test_rule_fixed_indegree(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 50)
+    b = ngpu.Create("aeif_cond_beta", 20)
+    ngpu.Connect(a, b, {"rule": "fixed_indegree", "indegree": 13},
+                 {"weight": 1.0, "delay": 1.0})
+    c = _conns(ngpu, a, b)
+    assert len(c) == 20 * 13
+    for j in range(20):
+        srcs = [s for s, t in c if t == b[j]]
+        assert len(srcs) == 13
+        assert len(set(srcs)) == 13          # without replacement
+        assert all(a[0] <= s < a[0] + 50 for s in srcs)
+
+
+def test_rule_fixed_outdegree(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 20)
+    b = ngpu.Create("aeif_cond_beta", 50)
+    ngpu.Connect(a, b, {"rule": "fixed_outdegree", "outdegree": 11},
+                 {"weight": 1.0, "delay": 1.0})
+    c = _conns(ngpu, a, b)
+    assert len(c) == 20 * 11
+    for i in range(20):
+        tg = [t for s, t in c if s == a[i]]
+        assert len(tg) == 11 and len(set(tg)) == 11
+
+
+def test_rule_fixed_total_number(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 30)
+    b = ngpu.Create("aeif_cond_beta", 40)
+    ngpu.Connect(a, b, {"rule": "fixed_total_number", "total_num": 321},
+                 {"weight": 1.0, "delay": 1.0})
+    c = _conns(ngpu, a, b)
+    assert len(c) == 321
+    assert all(a[0] <= s < a[0] + 30 and b[0] <= t < b[0] + 40 for s, t in c)
+
+
+def test_weight_delay_distributions(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 100)
+    ngpu.Connect(a, a, {"rule": "fixed_indegree", "indegree": 50},
+                 {"weight": {"distribution": "normal_clipped", "mu": 1.0,
+                             "sigma": 0.5, "low": 0.5, "high": 1.5},
+                  "delay": {"distribution": "normal_clipped", "mu": 2.0,
+                            "sigma": 1.0, "low": 0.5, "high": 3.0}})
+    st = ngpu.GetStatus(ngpu.GetConnections(a, a))
+    w = np.array([d["weight"] for d in st])
+    d = np.array([d["delay"] for d in st])
+    assert w.min() >= 0.5 and w.max() <= 1.5
+    assert d.min() >= 0.5 - 1e-6 and d.max() <= 3.0 + 1e-6
+    # delays are stored as whole time steps
+    steps = d / 0.1
+    assert np.allclose(steps, np.round(steps), atol=1e-3)
+    assert abs(w.mean() - 1.0) < 0.05
+
+
+def test_connection_status_fields(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 2, 3)
+    sg = ngpu.CreateSynGroup("stdp")
+    ngpu.Connect([a[0]], [a[1]], {"rule": "one_to_one"},
+                 {"weight": 2.5, "delay": 0.75, "receptor": 2,
+                  "synapse_group": sg})
+    (d,) = ngpu.GetStatus(ngpu.GetConnections(a[0], a[1]))
+    assert d["port"] == 2 and d["syn"] == sg.i_syn_group
+    assert d["weight"] == 2.5
+    # delay rounded to whole steps: round(0.75 / 0.1) = 8 steps
+    assert abs(d["delay"] - 0.8) < 1e-6
+
+
+# ---------------------------------------------------------------------
+# errors (src/connect.cu, src/neurongpu.cu messages)
+# ---------------------------------------------------------------------
+def test_error_delay_below_resolution(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 2)
+    with pytest.raises(ValueError):
+        ngpu.Connect(a, a, {"rule": "all_to_all"},
+                     {"weight": 1.0, "delay": 0.01})
+
+
+def test_error_port_out_of_range(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 2, 2)
+    with pytest.raises(ValueError):
+        ngpu.Connect(a, a, {"rule": "all_to_all"},
+                     {"weight": 1.0, "delay": 1.0, "receptor": 2})
+
+
+def test_error_unknown_model(ngpu):
+    with pytest.raises(ValueError):
+        ngpu.Create("no_such_model", 3)
+
+
+def test_error_unknown_parameter(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 2)
+    with pytest.raises(ValueError):
+        ngpu.SetStatus(a, {"no_such_param": 1.0})
+
+
+def test_error_state_cleared_after_success(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 2)
+    with pytest.raises(ValueError):
+        ngpu.SetStatus(a, {"no_such_param": 1.0})
+    ngpu.SetStatus(a, {"V_m": -60.0})
+    assert ngpu.GetStatus(a, "V_m") == [-60.0, -60.0]
+
+
+# ---------------------------------------------------------------------
+# parameters, names, defaults
+# ---------------------------------------------------------------------
+def test_aeif_cond_beta_defaults(ngpu):
+    # src/aeif_cond_beta.cu:25-56
+    a = ngpu.Create("aeif_cond_beta", 1, 2)
+    st = ngpu.GetStatus(a[0])
+    want = {"V_th": -50.4, "Delta_T": 2.0, "g_L": 30.0, "E_L": -70.6,
+            "C_m": 281.0, "a": 4.0, "b": 80.5, "tau_w": 144.0, "I_e": 0.0,
+            "V_peak": 0.0, "V_reset": -60.0, "t_ref": 0.0}
+    for k, v in want.items():
+        assert st[k] == float(np.float32(v)), k
+    assert st["E_rev"] == [0.0, 0.0]
+    assert st["tau_decay"] == [20.0, 20.0]
+    assert st["tau_rise"] == [2.0, 2.0]
+
+
+def test_names(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 1, 2)
+    assert ngpu.GetScalVarNames(a[0]) == ["V_m", "w"]
+    assert ngpu.GetPortVarNames(a[0]) == ["g", "g1"]
+    assert ngpu.GetPortParamNames(a[0]) == ["E_rev", "tau_rise", "tau_decay",
+                                            "g0"]
+    assert ngpu.GetScalParamNames(a[0])[:13] == [
+        "V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w", "I_e",
+        "V_peak", "V_reset", "t_ref", "refractory_step"]
+
+
+def test_array_setters(ngpu):
+    a = ngpu.Create("aeif_cond_beta", 5)
+    vals = np.linspace(-70.0, -60.0, 5).astype(np.float32)
+    ngpu.SetStatus(a, {"V_m": {"array": vals}})
+    assert np.array_equal(np.float32(ngpu.GetStatus(a, "V_m")), vals)
+    ngpu.SetStatus(a, {"V_m": {"distribution": "normal_clipped", "mu": 0.0,
+                               "sigma": 3.0, "low": -1.0, "high": 1.0}})
+    v = np.array(ngpu.GetStatus(a, "V_m"))
+    assert v.min() >= -1.0 and v.max() <= 1.0
+
+
+def test_random_helpers(ngpu):
+    u = np.array(ngpu.RandomUniform(10000))
+    assert u.min() >= 0.0 and u.max() <= 1.0
+    assert abs(u.mean() - 0.5) < 0.02
+    n = np.array(ngpu.RandomNormal(10000, 1.0, 2.0))
+    assert abs(n.mean() - 1.0) < 0.1 and abs(n.std() - 2.0) < 0.1
+    c = np.array(ngpu.RandomNormalClipped(10000, 0.0, 1.0, -0.5, 0.5))
+    assert c.min() >= -0.5 and c.max() <= 0.5
+
+
+def test_kernel_status(ngpu):
+    ngpu.SetTimeResolution(0.05)
+    # stored as float (src/neurongpu.h: float time_resolution_)
+    assert ngpu.GetTimeResolution() == float(np.float32(0.05))
+    ngpu.SetMaxSpikeBufferSize(30)
+    assert ngpu.GetMaxSpikeBufferSize() == 30

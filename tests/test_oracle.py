@@ -1,0 +1,359 @@
+"""Pins the CPU oracle before it is trusted as the checker of the HIP path.
+
+Each oracle function is checked against something it does not compute
+itself: the reference's golden vectors (NEST traces, STDP scenarios,
+spike-detector timing), published known-answer tests (Philox), library
+routines (scipy's Poisson CDF, matrix exponential, root finder), closed
+forms (beta-peak normalisation, STDP rule, AdEx fixed point) and
+convergence order.  Runs on the CPU only."""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.optimize
+import scipy.stats
+
+import netbuild as nb
+from oracle import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------------
+# Philox4x32-10: Random123 known-answer vectors (kat_vectors, philox4x32_10)
+# ---------------------------------------------------------------------
+@pytest.mark.parametrize("ctr,key,want", [
+    ([0, 0, 0, 0], [0, 0],
+     [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2,
+     [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+     [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_kat(ctr, key, want):
+    assert list(po.philox4x32_10(ctr, key)) == want
+
+
+# ---------------------------------------------------------------------
+# Poisson sampler: table of 32-bit CDF thresholds + inversion
+# ---------------------------------------------------------------------
+@pytest.mark.parametrize("lam", [1e-4, 0.01, 0.1, 0.8, 2.0, 8.0, 40.0])
+def test_poisson_table_matches_cdf(lam):
+    tab = po.poisson_table(lam)
+    assert tab is not None
+    k = np.arange(len(tab))
+    cdf = scipy.stats.poisson.cdf(k, lam)
+    # tab[k] = ceil(2^32 F(k)) - 1, clamped to the u32 range
+    want = np.minimum(np.ceil(cdf * 2.0 ** 32) - 1, 2.0 ** 32 - 1)
+    assert np.all(np.abs(tab.astype(np.float64) - want) <= 2.0)
+    assert tab[-1] == 0xFFFFFFFF
+    assert np.all(np.diff(tab.astype(np.int64)) >= 0)
+
+
+def test_poisson_sample_boundaries():
+    L = po.lib()
+    tab = po.poisson_table(0.8)
+    t = np.ascontiguousarray(tab, dtype=np.uint32)
+    for k in range(len(tab) - 1):
+        if tab[k] == tab[k + 1]:
+            continue
+        assert L.orc_poisson_sample(t.ctypes.data, int(tab[k])) == k
+        assert L.orc_poisson_sample(t.ctypes.data, int(tab[k]) + 1) == k + 1
+    assert L.orc_poisson_sample(t.ctypes.data, 0) == 0
+    assert L.orc_poisson_sample(t.ctypes.data, 0xFFFFFFFF) <= len(tab) - 1
+
+
+def test_poisson_zero_and_too_large_rate():
+    L = po.lib()
+    t0 = np.ascontiguousarray(po.poisson_table(0.0), dtype=np.uint32)
+    for u in (0, 12345, 0xFFFFFFFF):
+        assert L.orc_poisson_sample(t0.ctypes.data, u) == 0
+    assert po.poisson_table(400.0) is None      # table longer than 256
+
+
+def test_poisson_counts_distribution():
+    """Counts drawn with the Philox counter layout of the direct
+    connections follow Poisson(lambda) (chi-square, p > 1e-4)."""
+    L = po.lib()
+    lam = 0.8
+    t = np.ascontiguousarray(po.poisson_table(lam), dtype=np.uint32)
+    n = 200000
+    counts = np.zeros(12, dtype=np.int64)
+    for i in range(n):
+        u = int(po.philox4x32_10([i % 1000, i // 1000, 0, 0x504F4953],
+                                 [1234, 0])[0])
+        c = L.orc_poisson_sample(t.ctypes.data, u)
+        counts[min(c, 11)] += 1
+    pmf = scipy.stats.poisson.pmf(np.arange(11), lam)
+    exp = np.append(pmf, 1.0 - pmf.sum()) * n
+    mask = exp > 5
+    chi2 = np.sum((counts[mask] - exp[mask]) ** 2 / exp[mask])
+    p = scipy.stats.chi2.sf(chi2, mask.sum() - 1)
+    assert p > 1e-4
+
+
+# ---------------------------------------------------------------------
+# own exp / log (FP32, shared with the kernels)
+# ---------------------------------------------------------------------
+def _ulp_err(got, ref):
+    got = np.float32(got)
+    spacing = np.spacing(np.abs(np.float32(ref))).astype(np.float64)
+    return abs(np.float64(got) - ref) / spacing
+
+
+def test_expf_accuracy():
+    L = po.lib()
+    xs = np.concatenate([np.linspace(-87.0, 88.0, 200001),
+                         np.linspace(-1.0, 1.0, 20001),
+                         [0.0, -0.0, 1e-8, -1e-8]]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        ref = math.exp(float(x))
+        worst = max(worst, _ulp_err(L.orc_expf(float(x)), ref))
+    assert worst <= 2.0
+    assert L.orc_expf(0.0) == 1.0
+    assert L.orc_expf(-200.0) == 0.0
+    assert math.isinf(L.orc_expf(200.0))
+
+
+def test_logf_accuracy():
+    L = po.lib()
+    xs = np.concatenate([np.geomspace(1e-30, 1e30, 100001),
+                         np.linspace(0.5, 2.0, 20001)]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        ref = math.log(float(x))
+        if abs(ref) < 1e-6:
+            continue
+        worst = max(worst, _ulp_err(L.orc_logf(float(x)), ref))
+    assert worst <= 2.0
+    assert L.orc_logf(1.0) == 0.0
+
+
+# ---------------------------------------------------------------------
+# STDP rule (src/stdp.cu:24-51) against the reference's test formula
+# ---------------------------------------------------------------------
+def test_stdp_rule_closed_form():
+    L = po.lib()
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        p = {"tau_plus": rng.uniform(5, 40), "tau_minus": rng.uniform(5, 40),
+             "lambda": rng.uniform(1e-3, 0.1), "alpha": rng.uniform(0.5, 2),
+             "mu_plus": rng.choice([0.0, 0.5, 1.0, 2.0]),
+             "mu_minus": rng.choice([0.0, 0.5, 1.0, 2.0]),
+             "Wmax": rng.uniform(1, 100)}
+        w = float(np.float32(rng.uniform(0, p["Wmax"])))
+        dt = float(np.float32(rng.uniform(-50, 50)))
+        par = np.array([p[k] for k in po.STDP_PARAM], dtype=np.float32)
+        got = L.orc_stdp_update(w, dt, par.ctypes.data)
+        pf = {k: float(v) for k, v in zip(po.STDP_PARAM, par)}
+        want = nb.stdp_closed_form(w, [dt], pf)
+        assert abs(got - want) <= 2e-6 * max(1.0, abs(want)), (p, w, dt)
+
+
+def test_stdp_clamps():
+    L = po.lib()
+    par = np.array([20, 20, 0.5, 1, 1, 1, 10], dtype=np.float32)
+    assert L.orc_stdp_update(9.99, 0.1, par.ctypes.data) <= 10.0
+    assert L.orc_stdp_update(0.01, -0.1, par.ctypes.data) >= 0.0
+
+
+@pytest.mark.parametrize("case", [str(i) for i in range(1, 11)])
+def test_stdp_reference_cases(case):
+    """python/test/test_stdp/cases/case<k>.py: final weight equals the
+    closed-form sequence of updates the reference test expects."""
+    cs = nb.load_stdp_cases()
+    c = cs["cases"][case]
+    spec = nb.stdp_case_spec(c, cs["common"])
+    want = nb.stdp_closed_form(cs["common"]["weight"], c["Dt"], cs["common"])
+    for prec in ("f32", "f64"):
+        net, _ = nb.build_oracle(spec, prec)
+        net.run(int(round(cs["common"]["sim_time"] / 0.1)))
+        src, tgt, port, d, w = net.connections()
+        (got,) = w[(src == 0) & (tgt == 1)]
+        assert abs(float(got) - want) <= 1e-6 * want, (prec, got, want)
+
+
+# ---------------------------------------------------------------------
+# calibration closed forms
+# ---------------------------------------------------------------------
+@pytest.mark.parametrize("tr,td", [(0.2, 2.0), (0.5, 10.0), (2.0, 20.0),
+                                   (20.0, 40.0), (5.0, 5.0)])
+def test_beta_g0_normalises_peak(tr, td):
+    """g1(0) = g0 after a unit-weight spike; the conductance
+    g(t) = g0 (e^-t/td - e^-t/tr)/(1/tr - 1/td) (t e^-t/td if tr = td)
+    must peak at 1 (src/aeif_cond_beta.cu:66-85)."""
+    L = po.lib("f64")
+    g0 = L.orc_beta_g0(tr, td)
+    t = np.linspace(1e-6, 10 * td, 2000001)
+    if tr == td:
+        g = g0 * t * np.exp(-t / td)
+    else:
+        g = g0 * (np.exp(-t / td) - np.exp(-t / tr)) / (1 / tr - 1 / td)
+    assert abs(g.max() - 1.0) < 1e-9
+
+
+@pytest.mark.parametrize("h,tm,cm,tex,tin", [(0.1, 10.0, 250.0, 2.0, 8.0),
+                                             (0.1, 10.0, 250.0, 0.5, 0.5),
+                                             (0.05, 20.0, 100.0, 3.0, 1.0)])
+def test_iaf_propagators_match_expm(h, tm, cm, tex, tin):
+    """Exact integration of (I_ex, I_in, V): propagators equal the matrix
+    exponential of the linear system (src/iaf_psc_exp.cu:53-92)."""
+    L = po.lib("f64")
+    out = np.zeros(6, dtype=np.float32)
+    L.orc_iaf_propagators(h, tm, cm, tex, tin, out.ctypes.data)
+    P20, P11ex, P11in, P21ex, P21in, P22 = [float(v) for v in out]
+    A = np.array([[-1 / tex, 0, 0], [0, -1 / tin, 0],
+                  [1 / cm, 1 / cm, -1 / tm]])
+    E = scipy.linalg.expm(A * h)
+    want = [tm / cm * (1 - math.exp(-h / tm)), E[0, 0], E[1, 1], E[2, 0],
+            E[2, 1], E[2, 2]]
+    got = [P20, P11ex, P11in, P21ex, P21in, P22]
+    for g, w in zip(got, want):
+        assert abs(g - w) <= 2e-7 * max(abs(w), 1e-12) + 1e-12
+
+
+# ---------------------------------------------------------------------
+# RK5 (Cash-Karp) order and error control
+# ---------------------------------------------------------------------
+def test_rk5_fifth_order():
+    L = po.lib("f64")
+    errs = []
+    for n in (32, 64, 128, 256):
+        y = L.orc_rk5_linear_fixed(-2.0, 1.0, 2.0, n)
+        errs.append(abs(y - math.exp(-4.0)))
+    ratios = [errs[i] / errs[i + 1] for i in range(len(errs) - 1)]
+    # global error ~ h^5: halving h divides it by ~32 (asymptotic range;
+    # a dropped or wrong tableau term shows up as order <= 4, ratio <= 16)
+    for r in ratios:
+        assert 27.0 < r < 34.0, ratios
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-6, 1e-9])
+def test_rk5_adaptive_tolerance(eps):
+    L = po.lib("f64")
+    na = np.zeros(1, np.int64)
+    nr = np.zeros(1, np.int64)
+    y = L.orc_rk5_linear_adaptive(-3.0, 1.0, 5.0, 0.01, 1e-8, eps,
+                                  na.ctypes.data, nr.ctypes.data)
+    want = math.exp(-15.0)
+    # local error per step <= eps * (|y| + |h y'| + 0.1) scale; globally
+    # bounded by steps * eps * (1 + 0.1)
+    assert abs(y - want) <= na[0] * eps * 1.2
+    assert na[0] > 0
+
+
+# ---------------------------------------------------------------------
+# neuron models against the reference's golden NEST traces
+# ---------------------------------------------------------------------
+@pytest.mark.parametrize("model", ["aeif_cond_beta", "aeif_cond_alpha",
+                                   "iaf_psc_exp"])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_nest_golden_trace_oracle(model, prec):
+    """python/test/test_<model>.py: NEST row i <-> record row 10 i + 20,
+    std(dV)/|mean V| <= 5e-4 (aeif) or 5e-5 (iaf_psc_exp)."""
+    spec = nb.nest_spec(model)
+    net, _ = nb.build_oracle(spec, prec)
+    net.run(8000)
+    V = net.records()[:, 1]
+    if model == "iaf_psc_exp":
+        V = V - 70.0
+    nest = np.loadtxt(os.path.join(GOLDEN, "%s_nest.txt" % model),
+                      delimiter="\t")
+    dV = np.array([V[i * 10 + 20] - nest[i, 1] for i in range(len(nest))])
+    rmse = np.std(dV) / abs(np.mean(V))
+    tol = 5e-5 if model == "iaf_psc_exp" else 5e-4
+    assert rmse <= tol
+
+
+def test_aeif_fixed_point():
+    """Constant I_e below rheobase: V, w converge to the root of
+    dV = dw = 0 with w* = a (V* - E_L)."""
+    I_e = 300.0
+    spec = {"groups": [{"model": "aeif_cond_beta", "n": 1, "n_port": 1,
+                        "params": {"I_e": I_e}}],
+            "records": [(0, "V_m", 0), (0, "w", 0)]}
+    net, _ = nb.build_oracle(spec, "f64")
+    net.run(30000)                              # 3 s, ~20 tau_w
+    r = net.records()
+    V, W = r[-1, 1], r[-1, 2]
+    gL, EL, VT, DT, a = 30.0, -70.6, -50.4, 2.0, 4.0
+
+    def f(v):
+        return (-gL * (v - EL) + gL * DT * math.exp((v - VT) / DT)
+                - a * (v - EL) + I_e)
+    vstar = scipy.optimize.brentq(f, -70.0, -55.0, xtol=1e-13)
+    assert abs(V - vstar) < 1e-6 * abs(vstar)
+    assert abs(W - a * (vstar - EL)) < 1e-5 * abs(a * (vstar - EL))
+
+
+def test_rk5_tolerance_convergence():
+    """The same neuron at tolerance 1e-6 and 1e-9: subthreshold traces agree
+    to the looser tolerance's accuracy; a spiking neuron keeps its spike
+    count and the tighter tolerance takes more RK5 attempts."""
+    def run(I_e, tol, steps):
+        spec = {"groups": [{"model": "aeif_cond_beta", "n": 1, "n_port": 1,
+                            "params": {"I_e": I_e}, "rk5": {"rk5_tol": tol}}],
+                "records": [(0, "V_m", 0)]}
+        net, _ = nb.build_oracle(spec, "f64")
+        net.run(steps)
+        return net
+    va = run(400.0, 1e-6, 5000).records()[:, 1]
+    vb = run(400.0, 1e-9, 5000).records()[:, 1]
+    assert np.max(np.abs(va - vb)) < 1e-4
+    a, b = run(1000.0, 1e-6, 5000), run(1000.0, 1e-9, 5000)
+    assert len(a.spikes()[0]) == len(b.spikes()[0]) > 5
+    ca, cb = a.counters(), b.counters()
+    assert (cb["rk5_accepted"] + cb["rk5_rejected"] >
+            ca["rk5_accepted"] + ca["rk5_rejected"])
+
+
+# ---------------------------------------------------------------------
+# spike routing timing (python/test/test_spike_detector.py)
+# ---------------------------------------------------------------------
+def test_delivery_delay_rows():
+    """A spike of the source at record row i reaches a relay with delay
+    10/20/30 ms at row i + 101/201/301, with the synaptic weight as
+    height (python/test/test_spike_detector.py:37-52)."""
+    spec = {"groups": [
+        {"model": "aeif_cond_beta", "n": 3, "n_port": 1,
+         "params": {"I_e": 1000.0}},
+        {"model": "parrot_neuron", "n": 3,
+         "params": {"hold_spike_height": 1.0}}],
+        "conns": [{"src": [0], "tgt": [3], "weight": 1.0, "delay": 10.0},
+                  {"src": [1], "tgt": [4], "weight": 2.0, "delay": 20.0},
+                  {"src": [2], "tgt": [5], "weight": 3.0, "delay": 30.0}],
+        "records": [(3, "V", 0), (4, "V", 0), (5, "V", 0)]}
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(10000)
+    rec = net.records()
+    st, nd, ct = net.spikes()
+    src_rows = sorted(set(int(s) + 1 for s, n in zip(st, nd) if n == 0))
+    assert len(src_rows) > 5
+    n_rows = rec.shape[0]
+    for i in src_rows:
+        for k, (off, h) in enumerate([(101, 1.0), (201, 2.0), (301, 3.0)]):
+            if i + off < n_rows:
+                assert rec[i + off, 1 + k] == h
+    # and nothing arrives elsewhere
+    for k, off in enumerate([101, 201, 301]):
+        rows = set(np.nonzero(rec[:, 1 + k])[0])
+        assert rows == set(i + off for i in src_rows if i + off < n_rows)
+
+
+# ---------------------------------------------------------------------
+# network statistics (python/test/test_brunel_list.py:11,132-148)
+# ---------------------------------------------------------------------
+def test_brunel_rate():
+    spec = nb.brunel_spec()
+    net, _ = nb.build_oracle(spec, "f32")
+    n_steps = 10000
+    net.run(n_steps, n_threads=8)
+    st, nd, ct = net.spikes()
+    n_neur = spec["groups"][0]["n"]
+    rate = np.sum(ct[nd < n_neur]) / n_neur / (n_steps * 0.1e-3)
+    # reference expectation 30.78 Hz for its own random network
+    assert abs(rate - 30.78) < 0.1 * 30.78, rate

@@ -219,6 +219,10 @@ class NeuronGPU {
   int SetNeuronIntVar(NodeSeq nodes, std::string var_name, int val) {
     return SetNeuronIntVar(nodes.i0, nodes.n, var_name, val);
   }
+  int SetNeuronIntVar(std::vector<int> nodes, std::string var_name,
+                      int val) {
+    return SetNeuronIntVar(nodes.data(), (int)nodes.size(), var_name, val);
+  }
   int SetNeuronVar(int i_node, int n_neuron, std::string var_name, float val);
   int SetNeuronVar(int *i_node, int n_neuron, std::string var_name, float val);
   int SetNeuronVar(int i_node, int n_neuron, std::string var_name, float *var,
@@ -232,6 +236,14 @@ class NeuronGPU {
                    int array_size) {
     return SetNeuronVar(nodes.i0, nodes.n, var_name, var, array_size);
   }
+  int SetNeuronVar(std::vector<int> nodes, std::string var_name, float val) {
+    return SetNeuronVar(nodes.data(), (int)nodes.size(), var_name, val);
+  }
+  int SetNeuronVar(std::vector<int> nodes, std::string var_name, float *var,
+                   int array_size) {
+    return SetNeuronVar(nodes.data(), (int)nodes.size(), var_name, var,
+                        array_size);
+  }
   int GetNeuronParamSize(int i_node, std::string param_name);
   int GetNeuronVarSize(int i_node, std::string var_name);
   float *GetNeuronParam(int i_node, int n_neuron, std::string param_name);
@@ -239,13 +251,25 @@ class NeuronGPU {
   float *GetNeuronParam(NodeSeq nodes, std::string param_name) {
     return GetNeuronParam(nodes.i0, nodes.n, param_name);
   }
+  float *GetNeuronParam(std::vector<int> nodes, std::string param_name) {
+    return GetNeuronParam(nodes.data(), (int)nodes.size(), param_name);
+  }
   float *GetArrayParam(int i_node, std::string param_name);
   int *GetNeuronIntVar(int i_node, int n_neuron, std::string var_name);
   int *GetNeuronIntVar(int *i_node, int n_neuron, std::string var_name);
+  int *GetNeuronIntVar(NodeSeq nodes, std::string var_name) {
+    return GetNeuronIntVar(nodes.i0, nodes.n, var_name);
+  }
+  int *GetNeuronIntVar(std::vector<int> nodes, std::string var_name) {
+    return GetNeuronIntVar(nodes.data(), (int)nodes.size(), var_name);
+  }
   float *GetNeuronVar(int i_node, int n_neuron, std::string var_name);
   float *GetNeuronVar(int *i_node, int n_neuron, std::string var_name);
   float *GetNeuronVar(NodeSeq nodes, std::string var_name) {
     return GetNeuronVar(nodes.i0, nodes.n, var_name);
+  }
+  float *GetNeuronVar(std::vector<int> nodes, std::string var_name) {
+    return GetNeuronVar(nodes.data(), (int)nodes.size(), var_name);
   }
   float *GetArrayVar(int i_node, std::string var_name);
   int IsNeuronScalParam(int i_node, std::string param_name);
@@ -286,8 +310,10 @@ class NeuronGPU {
   unsigned int *RandomInt(size_t n);
   float *RandomUniform(size_t n);
   float *RandomNormal(size_t n, float mean, float stddev);
+  // vstep defaults to 0 so that the reference's c++/examples/brunel_net.cpp,
+  // which passes five arguments, compiles
   float *RandomNormalClipped(size_t n, float mean, float stddev, float vmin,
-                             float vmax, float vstep);
+                             float vmax, float vstep = 0.0f);
 
   int Connect(int i_source_node, int i_target_node, unsigned char port,
               unsigned char syn_group, float weight, float delay);
@@ -331,6 +357,32 @@ class NeuronGPU {
   int RemoteConnect(int i_source_host, int *source, int n_source,
                     int i_target_host, int *target, int n_target,
                     ConnSpec &conn_spec, SynSpec &syn_spec);
+  int RemoteConnect(int i_source_host, NodeSeq source, int i_target_host,
+                    NodeSeq target, ConnSpec &conn_spec, SynSpec &syn_spec) {
+    return RemoteConnect(i_source_host, source.i0, source.n, i_target_host,
+                         target.i0, target.n, conn_spec, syn_spec);
+  }
+  int RemoteConnect(int i_source_host, NodeSeq source, int i_target_host,
+                    std::vector<int> target, ConnSpec &conn_spec,
+                    SynSpec &syn_spec) {
+    return RemoteConnect(i_source_host, source.i0, source.n, i_target_host,
+                         target.data(), (int)target.size(), conn_spec,
+                         syn_spec);
+  }
+  int RemoteConnect(int i_source_host, std::vector<int> source,
+                    int i_target_host, NodeSeq target, ConnSpec &conn_spec,
+                    SynSpec &syn_spec) {
+    return RemoteConnect(i_source_host, source.data(), (int)source.size(),
+                         i_target_host, target.i0, target.n, conn_spec,
+                         syn_spec);
+  }
+  int RemoteConnect(int i_source_host, std::vector<int> source,
+                    int i_target_host, std::vector<int> target,
+                    ConnSpec &conn_spec, SynSpec &syn_spec) {
+    return RemoteConnect(i_source_host, source.data(), (int)source.size(),
+                         i_target_host, target.data(), (int)target.size(),
+                         conn_spec, syn_spec);
+  }
   int BuildDirectConnections();
 
   std::vector<std::string> GetScalVarNames(int i_node);
@@ -365,6 +417,29 @@ class NeuronGPU {
   std::vector<ConnectionId> GetConnections(int *i_source, int n_source,
                                            int *i_target, int n_target,
                                            int syn_group = -1);
+  std::vector<ConnectionId> GetConnections(NodeSeq source, NodeSeq target,
+                                           int syn_group = -1) {
+    return GetConnections(source.i0, source.n, target.i0, target.n,
+                          syn_group);
+  }
+  std::vector<ConnectionId> GetConnections(std::vector<int> source,
+                                           NodeSeq target,
+                                           int syn_group = -1) {
+    return GetConnections(source.data(), (int)source.size(), target.i0,
+                          target.n, syn_group);
+  }
+  std::vector<ConnectionId> GetConnections(NodeSeq source,
+                                           std::vector<int> target,
+                                           int syn_group = -1) {
+    return GetConnections(source.i0, source.n, target.data(),
+                          (int)target.size(), syn_group);
+  }
+  std::vector<ConnectionId> GetConnections(std::vector<int> source,
+                                           std::vector<int> target,
+                                           int syn_group = -1) {
+    return GetConnections(source.data(), (int)source.size(), target.data(),
+                          (int)target.size(), syn_group);
+  }
 
   int CreateSynGroup(std::string model_name);
   int GetSynGroupNParam(int syn_group);

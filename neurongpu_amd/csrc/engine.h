@@ -91,6 +91,9 @@ struct Group {
   DBuf<float> d_sg_h;
   DBuf<int> d_spike_count, d_n_rec_times;
   DBuf<float> d_rec_times;
+  DBuf<unsigned char> d_strag;      // resume records of the two-pass update
+  DBuf<unsigned int> d_strag_n;     // [2][64] sub-list lengths
+  unsigned long long n_strag_launch = 0;
   // synchronisation between host mirror and device
   bool host_newer = true;   // host arrays changed since last upload
   bool dev_newer = false;   // device advanced since last download
@@ -137,6 +140,10 @@ class Engine {
   int remote_spike_height_flag = 0;
   unsigned long long rng_calls = 0;  // host random stream counter
 
+  // two-pass RK5 update (false: one neuron per lane until done)
+  bool rk5_two_pass = true;
+  int rk5_budget = 1;          // RK5 attempts per neuron in the first pass
+  int rk5_pass2_grid = 1024;   // blocks of the second pass
   bool calibrated = false;
   bool first_simulation = true;
   bool in_simulation = false;
@@ -176,6 +183,13 @@ class Engine {
 
   // device state
   hipStream_t stream = nullptr;
+  // second RK5 passes of this step, launched after all first passes
+  struct DeferredPass2 {
+    AeifArgs a;
+    int n_port;
+  };
+  std::vector<DeferredPass2> deferred_pass2;
+  bool fuse_delivery = true;  // delivery in the last second-pass launch
   int max_port = 1;
   int R = 0;
   long long ring_cap = 0;
@@ -223,8 +237,9 @@ class Engine {
   size_t ev_used = 0;
   double kernel_ms[8] = {0};
   long long kernel_cnt[8] = {0};
-  void TimeBegin(int kind);
-  void TimeEnd();
+  void TimeBegin(int kind, hipStream_t s = nullptr);
+  void TimeEnd(hipStream_t s = nullptr);
+  void SyncAll();
   void CollectTimes();
 
   // helpers

@@ -147,6 +147,97 @@ __device__ __forceinline__ void aeif_rhs(const AeifLocal<P> &p, float refr,
   }
 }
 
+// Start of an adaptive Cash-Karp step (src/rk5.h:57-69): first stage and
+// the error scale, which stay fixed over the step's retries.
+template <int P>
+__device__ __forceinline__ void aeif_rk5_begin(const AeifLocal<P> &p,
+                                               float refr,
+                                               const float (&y)[2 + 2 * P],
+                                               float h,
+                                               float (&k1)[2 + 2 * P],
+                                               float (&inv_scal)[2 + 2 * P]) {
+  constexpr int N = 2 + 2 * P;
+  aeif_rhs<P>(p, refr, y, k1);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float s = fabsf(y[i]) + fabsf(k1[i] * h);
+    s = s + CK::scal_min;
+    inv_scal[i] = 1.0f / s;
+  }
+}
+
+// One attempt of the step with size hh (src/rk5.h:70-135): stages 2-6,
+// the fifth-order solution in yn and the scaled error norm (before the
+// division by the tolerance).
+template <int P>
+__device__ __forceinline__ float aeif_rk5_attempt(
+    const AeifLocal<P> &p, float refr, const float (&y)[2 + 2 * P],
+    const float (&k1)[2 + 2 * P], const float (&inv_scal)[2 + 2 * P],
+    float hh, float (&yn)[2 + 2 * P]) {
+  constexpr int N = 2 + 2 * P;
+  float k2[N], k3[N], k4[N], k5[N], k6[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float t = CK::a21 * k1[i];
+    yn[i] = __builtin_fmaf(hh, t, y[i]);
+  }
+  aeif_rhs<P>(p, refr, yn, k2);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float t = CK::a31 * k1[i];
+    t = __builtin_fmaf(CK::a32, k2[i], t);
+    yn[i] = __builtin_fmaf(hh, t, y[i]);
+  }
+  aeif_rhs<P>(p, refr, yn, k3);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float t = CK::a41 * k1[i];
+    t = __builtin_fmaf(CK::a42, k2[i], t);
+    t = __builtin_fmaf(CK::a43, k3[i], t);
+    yn[i] = __builtin_fmaf(hh, t, y[i]);
+  }
+  aeif_rhs<P>(p, refr, yn, k4);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float t = CK::a51 * k1[i];
+    t = __builtin_fmaf(CK::a52, k2[i], t);
+    t = __builtin_fmaf(CK::a53, k3[i], t);
+    t = __builtin_fmaf(CK::a54, k4[i], t);
+    yn[i] = __builtin_fmaf(hh, t, y[i]);
+  }
+  aeif_rhs<P>(p, refr, yn, k5);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float t = CK::a61 * k1[i];
+    t = __builtin_fmaf(CK::a62, k2[i], t);
+    t = __builtin_fmaf(CK::a63, k3[i], t);
+    t = __builtin_fmaf(CK::a64, k4[i], t);
+    t = __builtin_fmaf(CK::a65, k5[i], t);
+    yn[i] = __builtin_fmaf(hh, t, y[i]);
+  }
+  aeif_rhs<P>(p, refr, yn, k6);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float t = CK::a71 * k1[i];
+    t = __builtin_fmaf(CK::a73, k3[i], t);
+    t = __builtin_fmaf(CK::a74, k4[i], t);
+    t = __builtin_fmaf(CK::a76, k6[i], t);
+    yn[i] = __builtin_fmaf(hh, t, y[i]);
+  }
+  float err = 0.0f;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    float t = CK::e1 * k1[i];
+    t = __builtin_fmaf(CK::e3, k3[i], t);
+    t = __builtin_fmaf(CK::e4, k4[i], t);
+    t = __builtin_fmaf(CK::e5, k5[i], t);
+    t = __builtin_fmaf(CK::e6, k6[i], t);
+    float v = fabsf((hh * t) * inv_scal[i]);
+    err = (err > v) ? err : v;
+  }
+  return err;
+}
+
 // One adaptive Cash-Karp step (src/rk5.h:57-147).  Returns false when the
 // attempt cap is reached.
 template <int P>
@@ -157,79 +248,15 @@ __device__ __forceinline__ bool aeif_rk5_step(const AeifLocal<P> &p, float refr,
                                               float eps, int &n_att,
                                               int &n_acc, int &n_rej) {
   constexpr int N = 2 + 2 * P;
-  float k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], yn[N], inv_scal[N];
+  float k1[N], yn[N], inv_scal[N];
   float err = 0.0f;
-  aeif_rhs<P>(p, refr, y, k1);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float s = fabsf(y[i]) + fabsf(k1[i] * h);
-    s = s + CK::scal_min;
-    inv_scal[i] = 1.0f / s;
-  }
+  aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
   for (;;) {
     float hh = h;
     if (hh > h_max) hh = h_max;
     if (hh < h_min) hh = h_min;
     h = hh;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      float t = CK::a21 * k1[i];
-      yn[i] = __builtin_fmaf(hh, t, y[i]);
-    }
-    aeif_rhs<P>(p, refr, yn, k2);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      float t = CK::a31 * k1[i];
-      t = __builtin_fmaf(CK::a32, k2[i], t);
-      yn[i] = __builtin_fmaf(hh, t, y[i]);
-    }
-    aeif_rhs<P>(p, refr, yn, k3);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      float t = CK::a41 * k1[i];
-      t = __builtin_fmaf(CK::a42, k2[i], t);
-      t = __builtin_fmaf(CK::a43, k3[i], t);
-      yn[i] = __builtin_fmaf(hh, t, y[i]);
-    }
-    aeif_rhs<P>(p, refr, yn, k4);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      float t = CK::a51 * k1[i];
-      t = __builtin_fmaf(CK::a52, k2[i], t);
-      t = __builtin_fmaf(CK::a53, k3[i], t);
-      t = __builtin_fmaf(CK::a54, k4[i], t);
-      yn[i] = __builtin_fmaf(hh, t, y[i]);
-    }
-    aeif_rhs<P>(p, refr, yn, k5);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      float t = CK::a61 * k1[i];
-      t = __builtin_fmaf(CK::a62, k2[i], t);
-      t = __builtin_fmaf(CK::a63, k3[i], t);
-      t = __builtin_fmaf(CK::a64, k4[i], t);
-      t = __builtin_fmaf(CK::a65, k5[i], t);
-      yn[i] = __builtin_fmaf(hh, t, y[i]);
-    }
-    aeif_rhs<P>(p, refr, yn, k6);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      float t = CK::a71 * k1[i];
-      t = __builtin_fmaf(CK::a73, k3[i], t);
-      t = __builtin_fmaf(CK::a74, k4[i], t);
-      t = __builtin_fmaf(CK::a76, k6[i], t);
-      yn[i] = __builtin_fmaf(hh, t, y[i]);
-    }
-    err = 0.0f;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      float t = CK::e1 * k1[i];
-      t = __builtin_fmaf(CK::e3, k3[i], t);
-      t = __builtin_fmaf(CK::e4, k4[i], t);
-      t = __builtin_fmaf(CK::e5, k5[i], t);
-      t = __builtin_fmaf(CK::e6, k6[i], t);
-      float v = fabsf((hh * t) * inv_scal[i]);
-      err = (err > v) ? err : v;
-    }
+    err = aeif_rk5_attempt<P>(p, refr, y, k1, inv_scal, hh, yn);
     err = err / eps;
     n_att++;
     if (err <= 1.0f || (double)hh <= h_min_acc) break;
@@ -360,6 +387,329 @@ __global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
   }
 }
 
+template <int P>
+__device__ __forceinline__ void aeif_load_params(const AeifArgs &A, int i,
+                                                 AeifLocal<P> &p) {
+  p.V_th = pf(A.p.V_th, i);
+  p.Delta_T = pf(A.p.Delta_T, i);
+  p.inv_Delta_T = pf(A.p.inv_Delta_T, i);
+  p.g_L = pf(A.p.g_L, i);
+  p.E_L = pf(A.p.E_L, i);
+  p.inv_C_m = pf(A.p.inv_C_m, i);
+  p.a = pf(A.p.a, i);
+  p.b = pf(A.p.b, i);
+  p.inv_tau_w = pf(A.p.inv_tau_w, i);
+  p.I_e = pf(A.p.I_e, i);
+  p.V_peak = pf(A.p.V_peak, i);
+  p.V_reset = pf(A.p.V_reset, i);
+  p.t_ref = pf(A.p.t_ref, i);
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    p.E_rev[q] = pf(A.p.E_rev[q], i);
+    p.inv_tau_rise[q] = pf(A.p.inv_tau_rise[q], i);
+    p.inv_tau_decay[q] = pf(A.p.inv_tau_decay[q], i);
+    p.g0[q] = pf(A.p.g0[q], i);
+  }
+}
+
+// Resume record of a neuron whose step did not finish within the attempt
+// budget of the first pass.  Its state (y, x, current h, refractory count)
+// is in the group arrays; h_begin is the step size the current RK5 step
+// started with, from which the second pass recomputes the step's first
+// stage and error scale exactly.
+constexpr int kStragLists = 64;  // sub-lists of resume records
+
+struct StragRec {
+  uint32_t i;
+  uint32_t nsp;
+  uint32_t n_att;
+  float h_begin;
+};
+
+// One RK5 attempt of a neuron plus the step-boundary logic of RK5Update
+// and ExternalUpdate (src/rk5.h:70-162, src/aeif_cond_beta_kernel.h:
+// 178-214).  Same arithmetic, in the same order, as aeif_rk5_step and the
+// loop of aeif_kernel.  Returns true when the neuron has reached the end of
+// the time step.
+template <int P>
+__device__ __forceinline__ bool aeif_attempt_and_update(
+    const AeifArgs &A, const AeifLocal<P> &p, float (&y)[2 + 2 * P],
+    const float (&k1)[2 + 2 * P], const float (&inv_scal)[2 + 2 * P],
+    double &x, float &h, float &refr, int &nsp, int &n_att, bool &need_k1,
+    int &n_acc, int &n_rej) {
+  constexpr int N = 2 + 2 * P;
+  const double x1 = A.x1;
+  float yn[N];
+  float h_max = (float)(x1 - x);
+  float hh = h;
+  if (hh > h_max) hh = h_max;
+  if (hh < A.h_min) hh = A.h_min;
+  h = hh;
+  float err = aeif_rk5_attempt<P>(p, refr, y, k1, inv_scal, hh, yn);
+  err = err / A.eps;
+  n_att++;
+  if (err <= 1.0f || (double)hh <= A.h_min_acc) {
+    n_acc++;
+    x += (double)h;
+    if (err > CK::err_min) {
+      h = (h * CK::coeff) * dev_pow_inc(err);
+    } else {
+      h = 5.0f * h;
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) y[k] = yn[k];
+  } else {
+    n_rej++;
+    if (n_att > kMaxAttemptsPerStep) {
+      x = x1;  // give up on this step
+    } else {
+      float h_new = (hh * CK::coeff) * dev_pow_dec(err);
+      float h_lo = 0.1f * hh;
+      h = (h_new > h_lo) ? h_new : h_lo;
+      return false;  // retry with the smaller step
+    }
+  }
+  bool end = (x >= x1 - (double)A.h_min);
+  if (y[0] < -1.0e3f) {
+    y[0] = p.V_reset;
+    y[1] = 0.0f;
+  } else if (y[1] < -1.0e6f || y[1] > 1.0e6f) {
+    y[0] = p.V_reset;
+    y[1] = 0.0f;
+  } else if (refr > 0.0f) {
+    y[0] = p.V_reset;
+    if (end) refr -= 1.0f;
+  } else if (y[0] >= p.V_peak) {
+    nsp++;
+    y[0] = p.V_reset;
+    y[1] = y[1] + p.b;
+    float r = roundf(p.t_ref / A.dt);
+    int ri = (int)r;
+    refr = (float)(ri < 0 ? 0 : ri);
+  }
+  if (!end) need_k1 = true;
+  return end;
+}
+
+template <int P>
+__device__ __forceinline__ void aeif_store_state(const AeifArgs &A, int i,
+                                                 const float (&y)[2 + 2 * P],
+                                                 double x, float h,
+                                                 float refr) {
+  A.x[i] = x;
+  A.h[i] = h;
+  A.refr[i] = refr;
+  A.V[i] = y[0];
+  A.w[i] = y[1];
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    A.g[q][i] = y[2 + 2 * q];
+    A.g1[q][i] = y[3 + 2 * q];
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void aeif_count(const AeifArgs &A, int n_acc,
+                                           int n_rej) {
+  int sa = n_acc, sr = n_rej;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sa += __shfl_xor(sa, off);
+    sr += __shfl_xor(sr, off);
+  }
+  if (lane_id() == 0 && A.cnt && (sa | sr)) {
+    unsigned long long *c = A.cnt + 2 * (blockIdx.x & (kCounterSlots - 1));
+    atomicAdd(c, (unsigned long long)sa);
+    atomicAdd(c + 1, (unsigned long long)sr);
+  }
+}
+
+// First pass: one neuron per lane, coalesced, at most A.budget RK5
+// attempts.  Neurons that need more (those that spike or sit near
+// threshold, which would otherwise hold their 63 wave neighbours idle)
+// save their state and are listed per block for the second pass.
+template <int P>
+__global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
+  constexpr int N = 2 + 2 * P;
+  __shared__ uint32_t s_n, s_base;
+  if (threadIdx.x == 0) s_n = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < kStragLists)
+    A.strag_n_next[threadIdx.x] = 0u;
+  __syncthreads();
+  uint32_t slot = 0xFFFFFFFFu;
+  StragRec rec;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  bool active = i < A.n;
+  int ii = active ? i : 0;
+  int node = A.i0 + ii;
+  int nsp = 0, n_acc = 0, n_rej = 0;
+  bool finished = false;
+  if (active) {
+    AeifLocal<P> p;
+    aeif_load_params<P>(A, i, p);
+    float y[N], k1[N], inv_scal[N];
+    y[0] = A.V[i];
+    y[1] = A.w[i];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      y[2 + 2 * q] = A.g[q][i];
+      y[3 + 2 * q] = A.g1[q][i];
+    }
+    float in[P];
+    gather_input<P>(A.fold, node, in);
+#pragma unroll
+    for (int q = 0; q < P; q++)
+      y[3 + 2 * q] = __builtin_fmaf(in[q], p.g0[q], y[3 + 2 * q]);
+    double x = A.x[i];
+    float h = A.h[i];
+    float refr = A.refr[i];
+    int n_att = 0;
+    bool need_k1 = true;
+    float h_begin = h;
+    for (;;) {
+      if (n_att >= A.budget) break;
+      if (need_k1) {
+        h_begin = h;
+        aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
+        need_k1 = false;
+      }
+      if (aeif_attempt_and_update<P>(A, p, y, k1, inv_scal, x, h, refr, nsp,
+                                     n_att, need_k1, n_acc, n_rej)) {
+        finished = true;
+        break;
+      }
+    }
+    aeif_store_state<P>(A, i, y, x, h, refr);
+    if (!finished) {
+      slot = atomicAdd(&s_n, 1u);
+      rec.i = (uint32_t)i;
+      rec.nsp = (uint32_t)nsp;
+      rec.n_att = (uint32_t)n_att;
+      rec.h_begin = need_k1 ? h : h_begin;
+    }
+  }
+  emit_spikes(A.out, active && finished, ii, node, nsp, (float)nsp);
+  aeif_count<P>(A, n_acc, n_rej);
+  // this block's resume records go to sub-list blockIdx % kStragLists,
+  // at an offset reserved with one atomic per block
+  __syncthreads();
+  const uint32_t list = blockIdx.x % kStragLists;
+  if (threadIdx.x == 0)
+    s_base = s_n ? atomicAdd(A.strag_n + list, s_n) : 0u;
+  __syncthreads();
+  if (slot != 0xFFFFFFFFu)
+    reinterpret_cast<StragRec *>(A.strag)[(size_t)list * A.strag_cap +
+                                          s_base + slot] = rec;
+}
+
+// Second pass: the listed neurons, run to the end of the step.  The
+// sub-lists are concatenated (prefix over their counts, in LDS) and lane g
+// of the grid takes entries g, g + G, ...; a lane moves on to its next
+// entry as soon as the current neuron is done, so lanes stay busy while
+// neurons need different numbers of attempts, and waves beyond the list
+// end exit at once.
+template <int P>
+__device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
+                                                uint32_t vblock,
+                                                uint32_t vgrid) {
+  constexpr int N = 2 + 2 * P;
+  __shared__ uint32_t s_pre[kStragLists + 1];
+  if (threadIdx.x < 64) {  // one wave: exclusive scan of the 64 lengths
+    uint32_t c = A.strag_n[threadIdx.x];
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t v = __shfl_up(incl, off);
+      if ((int)threadIdx.x >= off) incl += v;
+    }
+    s_pre[threadIdx.x] = incl - c;
+    if (threadIdx.x == 63) s_pre[kStragLists] = incl;
+  }
+  __syncthreads();
+  const uint32_t total = s_pre[kStragLists];
+  const uint32_t n_lanes = vgrid * kBlock;
+  uint32_t j = vblock * kBlock + threadIdx.x;
+  if (vblock * kBlock >= total) return;  // whole block beyond the list
+  const StragRec *list = reinterpret_cast<const StragRec *>(A.strag);
+  AeifLocal<P> p;
+  float y[N], k1[N], inv_scal[N];
+  double x = 0.0;
+  float h = 0.0f, refr = 0.0f;
+  int nsp = 0, n_att = 0, n_acc = 0, n_rej = 0;
+  int i = 0;
+  bool need_k1 = false, have = false;
+  for (;;) {
+    if (!have && j < total) {
+      // sub-list of entry j: last l with s_pre[l] <= j
+      int lo = 0, hi = kStragLists;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= j) lo = mid; else hi = mid;
+      }
+      StragRec r = list[(size_t)lo * A.strag_cap + (j - s_pre[lo])];
+      j += n_lanes;
+      i = (int)r.i;
+      nsp = (int)r.nsp;
+      n_att = (int)r.n_att;
+      aeif_load_params<P>(A, i, p);
+      y[0] = A.V[i];
+      y[1] = A.w[i];
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        y[2 + 2 * q] = A.g[q][i];
+        y[3 + 2 * q] = A.g1[q][i];
+      }
+      x = A.x[i];
+      h = A.h[i];
+      refr = A.refr[i];
+      aeif_rk5_begin<P>(p, refr, y, r.h_begin, k1, inv_scal);
+      need_k1 = false;
+      have = true;
+    }
+    if (__ballot(have) == 0ull) break;
+    bool finished = false;
+    if (have) {
+      if (need_k1) {
+        aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
+        need_k1 = false;
+      }
+      finished = aeif_attempt_and_update<P>(A, p, y, k1, inv_scal, x, h, refr,
+                                            nsp, n_att, need_k1, n_acc, n_rej);
+      if (finished) aeif_store_state<P>(A, i, y, x, h, refr);
+    }
+    emit_spikes(A.out, finished, finished ? i : 0, A.i0 + (finished ? i : 0),
+                nsp, (float)nsp);
+    if (finished) have = false;
+  }
+  aeif_count<P>(A, n_acc, n_rej);
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void aeif_pass2_kernel(AeifArgs A) {
+  aeif_pass2_body<P>(A, blockIdx.x, gridDim.x);
+}
+
+template <bool PLASTIC>
+__device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
+                                              uint32_t vblock,
+                                              uint32_t vgrid);
+
+// Second RK5 pass and the delivery of the same step in one launch: blocks
+// [0, n2) run the second pass, the others the delivery.  The two are
+// independent within a step when no synapse is plastic (the delivery reads
+// spikes of earlier steps and writes only the input accumulators), so the
+// latency-bound second pass and the atomic-bound delivery share the chip
+// without a second stream.
+template <int P>
+__global__ __launch_bounds__(kBlock) void aeif_tail_kernel(AeifArgs A,
+                                                           DeliveryArgs D,
+                                                           uint32_t n2) {
+  if (blockIdx.x < n2)
+    aeif_pass2_body<P>(A, blockIdx.x, n2);
+  else
+    delivery_body<false>(D, blockIdx.x - n2, gridDim.x - n2);
+}
+
 // ---------------------------------------------------------------------
 // iaf_psc_exp, exact integration (src/iaf_psc_exp.cu:94-119)
 // ---------------------------------------------------------------------
@@ -479,7 +829,9 @@ __global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
 constexpr int kMaxDelaySlots = 4096;
 
 template <bool PLASTIC>
-__global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
+__device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
+                                              uint32_t vblock,
+                                              uint32_t vgrid) {
   __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
   __shared__ uint32_t s_part[kBlock];
   const int D = A.D;
@@ -518,19 +870,24 @@ __global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
     if (tid == kBlock - 1) s_pre[D] = s_part[kBlock - 1];
   }
   __syncthreads();
-  if (blockIdx.x == 0 && tid == 0) A.ring_n[(A.gs + 1) % A.R] = 0u;
+  if (vblock == 0 && tid == 0) A.ring_n[(A.gs + 1) % A.R] = 0u;
   const uint32_t total = s_pre[D];
-  const uint32_t n_tiles = (total + 63u) / 64u;
   const uint32_t lane = lane_id();
-  const uint32_t wave = (uint32_t)(blockIdx.x * (kBlock / 64) + tid / 64);
-  const uint32_t n_waves = gridDim.x * (kBlock / 64);
+  const uint32_t wave = (uint32_t)(vblock * (kBlock / 64) + tid / 64);
+  const uint32_t n_waves = vgrid * (kBlock / 64);
+  // pairs per wave tile: the largest power of two in [1, 64] that still
+  // gives every wave of the grid a tile, so that steps with few spikes
+  // spread their synapses over many waves instead of a few long loops
+  uint32_t pt = 64;
+  while (pt > 1 && (total + pt - 1) / pt < n_waves) pt >>= 1;
+  const uint32_t n_tiles = (total + pt - 1) / pt;
   unsigned long long n_events = 0;
   for (uint32_t wt = wave; wt < n_tiles; wt += n_waves) {
-    uint32_t pidx = wt * 64u + lane;
+    uint32_t pidx = wt * pt + lane;
     uint32_t cnt = 0, mult = 0;
     unsigned long long start = 0;
     float height = 0.0f;
-    if (pidx < total) {
+    if (lane < pt && pidx < total) {
       // delay slot d of this pair: last d with s_pre[d] <= pidx
       int lo = 0, hi = D;  // s_pre[lo] <= pidx < s_pre[hi]
       while (hi - lo > 1) {
@@ -558,52 +915,68 @@ __global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
     uint32_t pre = incl - cnt;
     uint32_t E = __shfl(incl, 63);
     n_events += E;
-    for (uint32_t e0 = 0; e0 < E; e0 += 64u) {
-      uint32_t e = e0 + lane;
-      // owner lane j: last lane with pre[j] <= e
-      int j = 0;
+    // two 64-event chunks per iteration: both record loads are issued
+    // before either atomic
+    for (uint32_t e0 = 0; e0 < E; e0 += 128u) {
+      unsigned long long syn[2], r[2];
+      float hj[2];
+      uint32_t mj[2];
+      bool ok[2];
 #pragma unroll
-      for (int stp = 32; stp > 0; stp >>= 1) {
-        int cand = j + stp;
-        uint32_t pv = __shfl(pre, cand);
-        if (cand < 64 && pv <= e) j = cand;
+      for (int u = 0; u < 2; u++) {
+        uint32_t e = e0 + 64u * u + lane;
+        // owner lane j < pt: last lane with pre[j] <= e
+        int j = 0;
+        for (uint32_t stp = pt >> 1; stp > 0; stp >>= 1) {
+          int cand = j + (int)stp;
+          uint32_t pv = __shfl(pre, cand);
+          if (pv <= e) j = cand;
+        }
+        uint32_t pj = __shfl(pre, j);
+        uint32_t slo = __shfl((uint32_t)start, j);
+        uint32_t shi = __shfl((uint32_t)(start >> 32), j);
+        hj[u] = __shfl(height, j);
+        mj[u] = __shfl(mult, j);
+        ok[u] = e < E;
+        syn[u] = (((unsigned long long)shi << 32) | slo) + (e - pj);
+        r[u] = ok[u] ? A.rec[syn[u]] : 0ull;
       }
-      uint32_t pj = __shfl(pre, j);
-      uint32_t slo = __shfl((uint32_t)start, j);
-      uint32_t shi = __shfl((uint32_t)(start >> 32), j);
-      float hj = __shfl(height, j);
-      uint32_t mj = __shfl(mult, j);
-      if (e < E) {
-        unsigned long long syn =
-            (((unsigned long long)shi << 32) | slo) + (e - pj);
-        unsigned long long r = A.rec[syn];
-        uint32_t tp = (uint32_t)r;
-        float w = __uint_as_float((uint32_t)(r >> 32));
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (!ok[u]) continue;
+        uint32_t tp = (uint32_t)r[u];
+        float w = __uint_as_float((uint32_t)(r[u] >> 32));
         uint32_t tgt = tp & kTargetMask;
         uint32_t port = tp >> kPortShift;
-        unsafeAtomicAdd(A.acc + (long long)port * A.n_total + tgt, hj * w);
+        unsafeAtomicAdd(A.acc + (long long)port * A.n_total + tgt, hj[u] * w);
         if (PLASTIC) {
-          int sgp = A.syn_group[syn];
+          int sgp = A.syn_group[syn[u]];
           if (sgp > 0) {
             int type = A.syn_type[sgp - 1];
             const float *prm = A.stdp_param + 8 * (sgp - 1);
             float wn = w;
-            for (uint32_t rep = 0; rep < mj; rep++) {
-              A.spike_time[syn] = (uint16_t)(A.T & 0xffff);
+            for (uint32_t rep = 0; rep < mj[u]; rep++) {
+              A.spike_time[syn[u]] = (uint16_t)(A.T & 0xffff);
               long long Dt_int = A.T - A.last_rev[tgt];
               if (Dt_int > 0 && Dt_int < kMaxSynDt)
                 wn = dev_syn_update(type, wn, -A.dt * (float)Dt_int, prm);
             }
             if (wn != w)
-              reinterpret_cast<uint32_t *>(A.rec + syn)[1] = __float_as_uint(wn);
+              reinterpret_cast<uint32_t *>(A.rec + syn[u])[1] =
+                  __float_as_uint(wn);
           }
         }
       }
     }
   }
   if (lane == 0 && n_events) {
-    atomicAdd(A.events + (blockIdx.x & (kCounterSlots - 1)), n_events);
+    atomicAdd(A.events + (vblock & (kCounterSlots - 1)), n_events);
   }
+}
+
+template <bool PLASTIC>
+__global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
+  delivery_body<PLASTIC>(A, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------
@@ -695,10 +1068,41 @@ __global__ void fill_u16_kernel(uint16_t *p, unsigned long long n, uint16_t v) {
 template <int P>
 void launch_aeif_p(const AeifArgs &a, hipStream_t s) {
   int grid = (a.n + kBlock - 1) / kBlock;
-  if (grid > 0) hipLaunchKernelGGL(aeif_kernel<P>, dim3(grid), dim3(kBlock), 0, s, a);
+  if (grid <= 0) return;
+  if (a.do_step && a.strag) {
+    hipLaunchKernelGGL(aeif_pass1_kernel<P>, dim3(grid), dim3(kBlock), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(aeif_kernel<P>, dim3(grid), dim3(kBlock), 0, s, a);
+  }
+}
+
+template <int P>
+void launch_aeif_pass2_p(const AeifArgs &a, const DeliveryArgs *d,
+                         int d_grid, hipStream_t s) {
+  int grid2 = a.pass2_grid > 0 ? a.pass2_grid : 256;
+  if (d)
+    hipLaunchKernelGGL(aeif_tail_kernel<P>, dim3(grid2 + d_grid), dim3(kBlock),
+                       0, s, a, *d, (uint32_t)grid2);
+  else
+    hipLaunchKernelGGL(aeif_pass2_kernel<P>, dim3(grid2), dim3(kBlock), 0, s, a);
 }
 
 }  // namespace
+
+void LaunchAeifPass2(const AeifArgs &a, int n_port, const DeliveryArgs *d,
+                     int d_grid, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  switch (n_port) {
+#define NGPU_CASE(P) \
+  case P: launch_aeif_pass2_p<P>(a, d, d_grid, s); break;
+    NGPU_CASE(1) NGPU_CASE(2) NGPU_CASE(3) NGPU_CASE(4) NGPU_CASE(5)
+    NGPU_CASE(6) NGPU_CASE(7) NGPU_CASE(8) NGPU_CASE(9) NGPU_CASE(10)
+    NGPU_CASE(11) NGPU_CASE(12) NGPU_CASE(13) NGPU_CASE(14) NGPU_CASE(15)
+    NGPU_CASE(16) NGPU_CASE(17) NGPU_CASE(18) NGPU_CASE(19) NGPU_CASE(20)
+#undef NGPU_CASE
+    default: break;
+  }
+}
 
 void LaunchAeif(const AeifArgs &a, int n_port, void *stream) {
   hipStream_t s = (hipStream_t)stream;

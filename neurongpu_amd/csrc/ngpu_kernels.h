@@ -97,6 +97,16 @@ struct AeifArgs {
   SpikeOut out;
   unsigned long long *cnt;    // [kCounterSlots][2] accepted, rejected
   int do_step;                // 0: fold only (end-of-simulation flush)
+  // two-pass update: first pass with at most `budget` RK5 attempts per
+  // neuron, the rest finished by a second pass over 64 sub-lists of
+  // 16-byte resume records (strag: [64][strag_cap]); strag_n holds the
+  // sub-list lengths of this launch, strag_n_next those of the next launch
+  // (cleared by this one); strag == nullptr selects the single-pass kernel
+  void *strag;
+  unsigned int *strag_n, *strag_n_next;
+  unsigned long long strag_cap;
+  int budget;
+  int pass2_grid;
 };
 
 struct IafArgs {
@@ -189,6 +199,10 @@ struct RecordArgs {
 
 // launchers (kernels.hip); all asynchronous on `stream`
 void LaunchAeif(const AeifArgs &a, int n_port, void *stream);
+// second pass of the two-pass update; with d != nullptr the delivery of
+// the step (non-plastic) runs in the same launch on d_grid more blocks
+void LaunchAeifPass2(const AeifArgs &a, int n_port, const DeliveryArgs *d,
+                     int d_grid, void *stream);
 void LaunchIaf(const IafArgs &a, void *stream);
 void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);

@@ -1384,7 +1384,7 @@ def GetHotPathStats():
 
 
 KERNEL_KINDS = ("aeif_rk5", "iaf_psc_exp", "relay", "spike_generator",
-                "delivery", "stdp_post", "record", "unused")
+                "delivery", "stdp_post", "record", "aeif_rk5_pass2")
 
 
 def SetKernelTiming(on):

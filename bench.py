@@ -34,6 +34,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -133,6 +134,50 @@ def build_readme(ngpu, n=1000000, seed=1234):
     return neuron, (n_e + n_i) * (ce + ci)
 
 
+def build_c2(ngpu, scale=1.0, seed=2014):
+    """c2: layered cortical-column-shaped iaf_psc_exp network (the recipe of
+    neurongpu_amd.workloads.c2_spec, built with the library's own
+    fixed_total_number rule and weight/delay distributions)."""
+    from neurongpu_amd import workloads as wl
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetRandomSeed(seed)
+    rng = np.random.default_rng(seed)
+    sizes = [max(1, int(round(n * scale))) for n in wl.C2_SIZES]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    n_tot = int(off[-1])
+    neuron = ngpu.Create("iaf_psc_exp", n_tot)
+    ngpu.SetStatus(neuron, {"V_m_rel": {"array": rng.uniform(
+        0.0, 15.0, n_tot).astype(np.float32)}})
+    pg = ngpu.Create("poisson_generator", 8)
+    p = rng.uniform(0.01, 0.15, (8, 8))
+    pops = [neuron[int(off[j]):int(off[j + 1])] for j in range(8)]
+    t = time.time()
+    n_syn = 0
+    for j in range(8):
+        ngpu.SetStatus(pg[j:j + 1], {"rate": 8.0 * wl.C2_K_EXT[j]})
+        for i in range(8):
+            k = int(round(p[j, i] * sizes[i] * sizes[j]))
+            if k == 0:
+                continue
+            exc = i % 2 == 0
+            syn = {"weight": {"distribution": "normal",
+                              "mu": 87.8 if exc else -351.2,
+                              "sigma": 8.8 if exc else 35.2},
+                   "delay": {"distribution": "normal_clipped",
+                             "mu": 1.5 if exc else 0.8,
+                             "sigma": 0.75 if exc else 0.4,
+                             "low": 0.1, "high": 1.0e3},
+                   "receptor": 0 if exc else 1}
+            ngpu.Connect(pops[i], pops[j],
+                         {"rule": "fixed_total_number", "total_num": k}, syn)
+            n_syn += k
+        ngpu.Connect(pg[j:j + 1], pops[j], {"rule": "all_to_all"},
+                     {"weight": 87.8, "delay": 0.1, "receptor": 0})
+    log("  connections in %.1f s (%d synapses)" % (time.time() - t, n_syn))
+    return neuron, n_syn
+
+
 def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps):
     dt = ngpu.GetTimeResolution()
     ngpu.ActivateSpikeCount(neuron)
@@ -202,10 +247,27 @@ def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps):
     return res
 
 
-def roofline(res):
+def profiled_traffic(name="aeif_rk5_c1.md"):
+    """HBM bytes per launch of the first RK5 pass on c1, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled against a copy kernel);
+    None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"Scaled traffic: ([0-9.]+) MB per launch", txt)
+    return float(m.group(1)) * 1e6 if m else None
+
+
+def roofline(res, traffic=None):
     """Dominant kernel of the step and its algorithmic traffic."""
     km = res["kernel_ms"]
-    dom = max(km, key=lambda k: km[k]["total_ms"])
+    # candidates with an algorithmic byte count per launch: the first RK5
+    # pass (72 B per neuron-step) and the delivery (9 B per event); the
+    # second RK5 pass is latency-bound on the slowest neurons
+    cand = [k for k in ("aeif_rk5", "delivery") if k in km]
+    dom = max(cand, key=lambda k: km[k]["total_ms"])
     if dom == "delivery":
         per_launch = DELIVERY_BYTES_PER_EVENT * res["delivery_kernel"][
             "events_per_launch"]
@@ -218,7 +280,7 @@ def roofline(res):
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_us": round(km[dom]["avg_us"], 2),
-            "traffic": None}
+            "traffic": traffic if dom == "aeif_rk5" else None}
 
 
 def cpu_baseline(n_sample=100000, steps=500, warmup=100, seed=1234):
@@ -283,6 +345,10 @@ def main():
     ap.add_argument("--only-readme", action="store_true",
                     help="measure only the README network (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workloads", default="c2",
+                    help="further configurations to measure after c1 "
+                         "(comma list of c2; empty for none)")
+    ap.add_argument("--c2-scale", type=float, default=1.0)
     ap.add_argument("--cpu-neurons", type=int, default=100000)
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -335,7 +401,7 @@ def main():
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(c1["rk5_kernel"]["tflops"] /
                                    FP32_PEAK_TFLOPS, 4)},
-        "roofline": roofline(c1),
+        "roofline": roofline(c1, profiled_traffic()),
     }
     if not args.no_readme:
         ngpu.ResetKernel()
@@ -359,6 +425,25 @@ def main():
             "speedup_vs_reference": round(70.0 / rd["wall_s_per_bio_s"], 2),
             "kernels": rd["kernel_ms"],
             "roofline": roofline(rd)}
+        ngpu.ResetKernel()
+    if "c2" in args.workloads.split(","):
+        log("building c2 (%.3g x 77169 iaf_psc_exp neurons)" % args.c2_scale)
+        neuron, n_syn = build_c2(ngpu, args.c2_scale)
+        r2 = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
+                     args.timing_steps)
+        log("c2:", json.dumps(r2))
+        out["c2"] = {
+            "workload": "c2: 8 iaf_psc_exp populations (77169 neurons), "
+                        "p ~ U(0.01, 0.15), E w ~ N(87.8, 8.8) pA, "
+                        "I w = -4x, delays N(1.5, 0.75)/N(0.8, 0.4) ms, "
+                        "Poisson 8 Hz x K_ext, dt 0.1 ms",
+            "neurons": r2["n_neurons"], "synapses": r2["n_synapses"],
+            "wall_s_per_bio_s": round(r2["wall_s_per_bio_s"], 4),
+            "ms_per_step": round(r2["ms_per_step"], 4),
+            "syn_events_per_s": round(r2["syn_events_per_s"], 1),
+            "rate_hz": round(r2["rate_hz"], 3),
+            "kernels": r2["kernel_ms"],
+            "roofline": roofline(r2)}
         ngpu.ResetKernel()
     if not args.no_cpu_baseline:
         log("cpu baseline")

@@ -1,0 +1,171 @@
+"""Seeded synthetic networks of the benchmark configurations, as explicit
+specifications, and a builder that creates a specification through the
+public Python API (i.e. through the C ABI).
+
+A specification is a dict:
+  dt, seed,
+  groups:   [ {model, n, n_port, params{name: scalar | list | array},
+               vars{...}, rk5: {h_min_rel, h0_rel, rk5_tol}} ]
+  conns:    [ {src, tgt, weight, delay, port, syn_group} ]  (arrays, one
+              entry per synapse, connected one_to_one in the given order)
+  poisson:  [ {node, rate, origin, start, stop} ]
+  spike_gen:[ {node, times, heights} ]
+  stdp:     [ {param dict} ]        (synapse groups 1, 2, ...)
+  records:  [ (node, var, port) ]
+
+The same specification builds the CPU oracle's copy of a network in the
+tests, so both see identical nodes, parameters, synapses and Poisson
+streams.
+"""
+import numpy as np
+
+PORT_PARAMS = {"E_rev", "tau_rise", "tau_decay", "tau_syn", "g0"}
+
+
+def build_from_spec(ngpu, spec):
+    """Reset the kernel and create the network of `spec`; returns the node
+    sequences of the groups and the record ids."""
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetTimeResolution(spec.get("dt", 0.1))
+    ngpu.SetRandomSeed(spec.get("seed", 54321))
+    groups = []
+    for g in spec["groups"]:
+        ns = ngpu.Create(g["model"], g["n"], g.get("n_port", 1))
+        groups.append(ns)
+        for k, v in g.get("params", {}).items():
+            _set(ngpu, ns, k, v)
+        for k, v in g.get("vars", {}).items():
+            _set(ngpu, ns, k, v)
+        if "rk5" in g:
+            for k, v in g["rk5"].items():
+                ngpu.SetStatus(ns, {k: v})
+    for s in spec.get("stdp", []):
+        ngpu.CreateSynGroup("stdp", dict(s))
+    for p in spec.get("poisson", []):
+        ngpu.SetStatus([p["node"]], {"rate": p["rate"],
+                                     "origin": p.get("origin", 0.0),
+                                     "start": p.get("start", 0.0),
+                                     "stop": p.get("stop", 1.0e30)})
+    for sg in spec.get("spike_gen", []):
+        d = {"spike_times": list(map(float, sg["times"]))}
+        if sg.get("heights") is not None:
+            d["spike_heights"] = list(map(float, sg["heights"]))
+        ngpu.SetStatus([sg["node"]], d)
+    for c in spec.get("conns", []):
+        src = np.ascontiguousarray(c["src"], dtype=np.int32)
+        n = src.size
+        if n == 0:
+            continue
+        syn = {"weight": {"array": np.broadcast_to(
+            np.asarray(c["weight"], np.float32), n)},
+               "delay": {"array": np.broadcast_to(
+                   np.asarray(c["delay"], np.float32), n)},
+               "receptor": int(c.get("port", 0)),
+               "synapse_group": int(c.get("syn_group", 0))}
+        ngpu.Connect(src, np.ascontiguousarray(c["tgt"], dtype=np.int32),
+                     {"rule": "one_to_one"}, syn)
+    recs = []
+    for (node, var, port) in spec.get("records", []):
+        recs.append(ngpu.CreateRecord("", [var], [node], [port]))
+    return groups, recs
+
+
+def _set(ngpu, ns, k, v):
+    if k in PORT_PARAMS:
+        ngpu.SetStatus(ns, {k: list(map(float, v))})
+    elif np.ndim(v) == 0:
+        ngpu.SetStatus(ns, {k: float(v)})
+    else:
+        ngpu.SetStatus(ns, {k: {"array": np.asarray(v, np.float32)}})
+
+
+def _normal_clipped(rng, mu, sigma, low, n, high=np.inf):
+    x = rng.normal(mu, sigma, n)
+    bad = (x < low) | (x > high)
+    while bad.any():
+        x[bad] = rng.normal(mu, sigma, int(bad.sum()))
+        bad = (x < low) | (x > high)
+    return x.astype(np.float32)
+
+
+# ---------------------------------------------------------------------
+# c2: layered cortical-column-shaped network of iaf_psc_exp neurons
+# ---------------------------------------------------------------------
+C2_SIZES = [20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948]
+# external in-degrees of python/Potjans_2014_s/network_params.py:108,
+# clipped to the configuration's range [1600, 2100]
+C2_K_EXT = [1600, 1600, 2100, 1900, 2000, 1900, 2100, 2100]
+
+
+def c2_spec(scale=1.0, seed=2014, integer_weights=False, dt=0.1,
+            compensate=True):
+    """8 populations (E, I alternating) of iaf_psc_exp neurons with the
+    configuration's sizes times `scale`.  Connection probability p[j][i]
+    ~ U(0.01, 0.15) per (target j, source i) population pair gives
+    round(p N_i N_j) synapses with sources and targets drawn uniformly
+    (fixed total number, multapses allowed).  Excitatory weights
+    N(87.8, 8.8) pA on port 0, inhibitory -4 N(87.8, 8.8) pA on port 1;
+    delays N(1.5, 0.75) ms (E) and N(0.8, 0.4) ms (I) clipped to >= 0.1 ms.
+    One Poisson generator per population at 8 Hz x K_ext, weight 87.8 pA.
+    With integer_weights the weights are rounded to whole pA, so every
+    synaptic sum is exact in any order.  Below full size (scale < 1) the
+    in-degrees shrink with the population sizes; with `compensate` the
+    recurrent weights are divided by `scale` so that the mean recurrent
+    input per neuron stays that of the full network."""
+    rng = np.random.default_rng(seed)
+    sizes = [max(1, int(round(n * scale))) for n in C2_SIZES]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    n_tot = int(off[-1])
+    p = rng.uniform(0.01, 0.15, (8, 8))
+    v0 = rng.uniform(0.0, 15.0, n_tot).astype(np.float32)
+    conns = []
+    for j in range(8):
+        for i in range(8):
+            k = int(round(p[j, i] * sizes[i] * sizes[j]))
+            if k == 0:
+                continue
+            src = (off[i] + rng.integers(0, sizes[i], k)).astype(np.int32)
+            tgt = (off[j] + rng.integers(0, sizes[j], k)).astype(np.int32)
+            w = rng.normal(87.8, 8.8, k)
+            if compensate and scale < 1.0:
+                w = w / scale
+            exc = i % 2 == 0
+            if exc:
+                d = _normal_clipped(rng, 1.5, 0.75, 0.1, k)
+            else:
+                w = -4.0 * w
+                d = _normal_clipped(rng, 0.8, 0.4, 0.1, k)
+            if integer_weights:
+                w = np.round(w)
+            conns.append({"src": src, "tgt": tgt,
+                          "weight": w.astype(np.float32), "delay": d,
+                          "port": 0 if exc else 1})
+    w_ext = 88.0 if integer_weights else 87.8
+    gen0 = n_tot
+    for j in range(8):
+        conns.append({"src": np.full(sizes[j], gen0 + j, np.int32),
+                      "tgt": np.arange(off[j], off[j + 1], dtype=np.int32),
+                      "weight": np.float32(w_ext), "delay": np.float32(0.1),
+                      "port": 0})
+    return {
+        "dt": dt, "seed": seed,
+        "groups": [{"model": "iaf_psc_exp", "n": n_tot,
+                    "vars": {"V_m_rel": v0}},
+                   {"model": "poisson_generator", "n": 8}],
+        "poisson": [{"node": gen0 + j, "rate": 8.0 * C2_K_EXT[j]}
+                    for j in range(8)],
+        "conns": conns,
+        "populations": [(int(off[j]), int(off[j + 1])) for j in range(8)],
+    }
+
+
+def n_synapses(spec, exclude_generators=True):
+    gen_nodes = set(p["node"] for p in spec.get("poisson", []))
+    tot = 0
+    for c in spec["conns"]:
+        src = np.asarray(c["src"])
+        if exclude_generators and src.size and int(src[0]) in gen_nodes:
+            continue
+        tot += src.size
+    return tot

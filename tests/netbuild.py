@@ -14,63 +14,8 @@ A description is a dict:
 """
 import numpy as np
 
+from neurongpu_amd.workloads import build_from_spec as build_engine  # noqa: F401
 from oracle import pyoracle as po
-
-PORT_PARAMS = {"E_rev", "tau_rise", "tau_decay", "tau_syn", "g0"}
-
-
-def build_engine(ngpu, spec):
-    ngpu.ResetKernel()
-    ngpu.SetVerbosityLevel(0)
-    ngpu.SetTimeResolution(spec.get("dt", 0.1))
-    ngpu.SetRandomSeed(spec.get("seed", 54321))
-    groups = []
-    for g in spec["groups"]:
-        ns = ngpu.Create(g["model"], g["n"], g.get("n_port", 1))
-        groups.append(ns)
-        for k, v in g.get("params", {}).items():
-            _set_engine(ngpu, ns, k, v)
-        for k, v in g.get("vars", {}).items():
-            _set_engine(ngpu, ns, k, v)
-        if "rk5" in g:
-            for k, v in g["rk5"].items():
-                ngpu.SetStatus(ns, {k: v})
-    for s in spec.get("stdp", []):
-        ngpu.CreateSynGroup("stdp", dict(s))
-    for p in spec.get("poisson", []):
-        ngpu.SetStatus([p["node"]], {"rate": p["rate"],
-                                     "origin": p.get("origin", 0.0),
-                                     "start": p.get("start", 0.0),
-                                     "stop": p.get("stop", 1.0e30)})
-    for sg in spec.get("spike_gen", []):
-        d = {"spike_times": list(map(float, sg["times"]))}
-        if sg.get("heights") is not None:
-            d["spike_heights"] = list(map(float, sg["heights"]))
-        ngpu.SetStatus([sg["node"]], d)
-    for c in spec.get("conns", []):
-        src = np.asarray(c["src"], dtype=np.int32)
-        n = src.size
-        syn = {"weight": {"array": np.broadcast_to(
-            np.asarray(c["weight"], np.float32), n)},
-               "delay": {"array": np.broadcast_to(
-                   np.asarray(c["delay"], np.float32), n)},
-               "receptor": int(c.get("port", 0)),
-               "synapse_group": int(c.get("syn_group", 0))}
-        ngpu.Connect(src, np.asarray(c["tgt"], dtype=np.int32),
-                     {"rule": "one_to_one"}, syn)
-    recs = []
-    for (node, var, port) in spec.get("records", []):
-        recs.append(ngpu.CreateRecord("", [var], [node], [port]))
-    return groups, recs
-
-
-def _set_engine(ngpu, ns, k, v):
-    if k in PORT_PARAMS:
-        ngpu.SetStatus(ns, {k: list(map(float, v))})
-    elif np.ndim(v) == 0:
-        ngpu.SetStatus(ns, {k: float(v)})
-    else:
-        ngpu.SetStatus(ns, {k: {"array": np.asarray(v, np.float32)}})
 
 
 def build_oracle(spec, prec="f32"):
@@ -349,3 +294,42 @@ def stdp_net_spec(n_src=200, n_post=100, k_plastic=20, seed=99,
              "port": 1, "syn_group": 1},
         ],
     }
+
+
+def spike_stats(times_by_neuron, n_neurons, t0, t1, n_pairs=500, bin_ms=5.0,
+                seed=0):
+    """Population statistics of a spike raster: mean rate (spikes/s per
+    neuron), mean coefficient of variation of the inter-spike intervals
+    (neurons with >= 3 spikes) and mean pairwise Pearson correlation of
+    spike counts in bin_ms bins over n_pairs random neuron pairs."""
+    dur = (t1 - t0) * 1e-3
+    counts = np.array([len(ts) for ts in times_by_neuron], dtype=np.float64)
+    rate = counts.sum() / n_neurons / dur
+    cvs = []
+    for ts in times_by_neuron:
+        if len(ts) >= 3:
+            isi = np.diff(np.sort(ts))
+            if isi.mean() > 0:
+                cvs.append(isi.std() / isi.mean())
+    cv = float(np.mean(cvs)) if cvs else 0.0
+    nb = int(round((t1 - t0) / bin_ms))
+    rng = np.random.default_rng(seed)
+    cc = []
+    for _ in range(n_pairs):
+        a, b = rng.choice(n_neurons, 2, replace=False)
+        ha = np.histogram(times_by_neuron[a], bins=nb, range=(t0, t1))[0]
+        hb = np.histogram(times_by_neuron[b], bins=nb, range=(t0, t1))[0]
+        if ha.std() > 0 and hb.std() > 0:
+            cc.append(np.corrcoef(ha, hb)[0, 1])
+    corr = float(np.mean(cc)) if cc else 0.0
+    return {"rate": float(rate), "cv_isi": cv, "corr": corr}
+
+
+def oracle_times_by_neuron(net, dt, lo, hi):
+    st, nd, ct = net.spikes()
+    out = [[] for _ in range(hi - lo)]
+    for s, n, c in zip(st, nd, ct):
+        if lo <= n < hi:
+            t = float(np.float64(np.float32(dt)) * (s + 1))
+            out[n - lo] += [t] * int(c)
+    return [np.array(x) for x in out]

@@ -222,3 +222,55 @@ def test_stdp_network_bit_exact(ngpu):
         n_eq += 1
     log("plastic weights equal", n_eq)
     assert n_eq == 2000
+
+
+def test_network_statistics_vs_fp64(ngpu):
+    """Brunel network of the reference test at 10 000 neurons (order 2000),
+    500 ms: per-population firing rate and mean CV of the inter-spike
+    intervals within 2 % of the FP64 oracle, mean pairwise spike-count
+    correlation (5 ms bins) within 0.02 absolute.  Spike trains diverge
+    (chaotic dynamics, different rounding), so only the statistics are
+    compared; the first 100 ms are discarded."""
+    log("case network_statistics")
+    order, steps = 2000, 5000
+    spec = nb.brunel_spec(order=order)
+    n, ne = 5 * order, 4 * order
+    groups, _ = nb.build_engine(ngpu, spec)
+    ngpu.ActivateRecSpikeTimes(groups[0], 1000)
+    ngpu.Simulate(steps * 0.1)
+    eng = [np.array(ngpu.GetRecSpikeTimes(i), dtype=np.float64)
+           for i in range(n)]
+    net, _ = nb.build_oracle(spec, "f64")
+    net.run(steps, 16)
+    orc = nb.oracle_times_by_neuron(net, 0.1, 0, n)
+    t1 = steps * 0.1
+    for name, sl, pairs in (("E", slice(0, ne), 500),
+                            ("I", slice(ne, n), 200)):
+        k = sl.stop - sl.start
+        se = nb.spike_stats(eng[sl], k, 100.0, t1, n_pairs=pairs)
+        so = nb.spike_stats(orc[sl], k, 100.0, t1, n_pairs=pairs)
+        log(name, "gpu", se, "fp64 oracle", so)
+        assert abs(se["rate"] - so["rate"]) <= 0.02 * so["rate"]
+        assert abs(se["cv_isi"] - so["cv_isi"]) <= 0.02 * so["cv_isi"]
+        assert abs(se["corr"] - so["corr"]) <= 0.02
+
+
+def test_c2_iaf_network_bit_exact(ngpu):
+    """c2-shaped iaf_psc_exp network at 10 % size (7717 neurons, 5e6
+    synapses, 8 Poisson populations), weights rounded to whole pA so that
+    every synaptic sum is exact: spike trains equal the FP32 oracle's spike
+    for spike over 100 ms."""
+    from neurongpu_amd import workloads as wl
+    log("case c2_scaled")
+    spec = wl.c2_spec(scale=0.1, integer_weights=True)
+    n = spec["groups"][0]["n"]
+    groups, _ = nb.build_engine(ngpu, spec)
+    ngpu.ActivateRecSpikeTimes(groups[0], 200)
+    ngpu.Simulate(100.0)
+    eng = sorted(nb.engine_spikes(ngpu, groups[0]))
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(1000, 16)
+    orc = sorted(nb.oracle_spikes(net, 0.1, 0, n))
+    log("spikes engine", len(eng), "oracle", len(orc), net.counters())
+    assert len(orc) > 10000
+    assert eng == orc

@@ -357,3 +357,22 @@ def test_brunel_rate():
     rate = np.sum(ct[nd < n_neur]) / n_neur / (n_steps * 0.1e-3)
     # reference expectation 30.78 Hz for its own random network
     assert abs(rate - 30.78) < 0.1 * 30.78, rate
+
+
+def test_c2_spec_shape():
+    """c2 recipe: population sizes, synapse count ~ sum p N_i N_j, weight
+    signs by source population, delays >= 0.1 ms."""
+    from neurongpu_amd import workloads as wl
+    spec = wl.c2_spec(scale=0.05, integer_weights=True)
+    pops = spec["populations"]
+    assert [b - a for a, b in pops] == [
+        int(round(n * 0.05)) for n in wl.C2_SIZES]
+    for c in spec["conns"]:
+        src = np.asarray(c["src"])
+        if src[0] >= pops[-1][1]:
+            continue                      # Poisson generator
+        pop = [k for k, (a, b) in enumerate(pops) if a <= src[0] < b][0]
+        w = np.asarray(c["weight"])
+        assert (w > 0).all() if pop % 2 == 0 else (w < 0).all()
+        assert np.all(np.asarray(c["delay"]) >= 0.1)
+        assert np.all(w == np.round(w))

@@ -178,6 +178,48 @@ def build_c2(ngpu, scale=1.0, seed=2014):
     return neuron, n_syn
 
 
+def build_c3(ngpu, scale=1.0, seed=4321, k=500, w_in=0.25, w_p=0.3,
+             rate=8000.0, tol=1e-3):
+    """c3: aeif_cond_alpha network with STDP on E -> E (the recipe of
+    neurongpu_amd.workloads.c3_spec, built with the library's
+    fixed_indegree rule)."""
+    from neurongpu_amd import workloads as wl
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetRandomSeed(seed)
+    rng = np.random.default_rng(seed)
+    n = max(10, int(round(200000 * scale)))
+    n_e = int(round(0.8 * n))
+    k_e, k_i = int(round(0.8 * k)), k - int(round(0.8 * k))
+    neuron = ngpu.Create("aeif_cond_alpha", n, 2)
+    exc, inh = neuron[0:n_e], neuron[n_e:n]
+    ngpu.SetStatus(neuron, dict(wl.C3_NEURON))
+    ngpu.SetStatus(neuron, {"V_m": {"array": rng.uniform(
+        -70.0, -55.0, n).astype(np.float32)}, "rk5_tol": tol})
+    pg = ngpu.Create("poisson_generator", n)
+    ngpu.SetStatus(pg, {"rate": rate})
+    sg = ngpu.CreateSynGroup("stdp", dict(wl.C3_STDP))
+    t = time.time()
+
+    def arrays(m):
+        return ({"array": rng.uniform(0.02, 0.06, m).astype(np.float32)},
+                {"array": (rng.integers(1, 21, m) * 0.1).astype(np.float32)})
+    w, d = arrays(n_e * k_e)
+    ngpu.Connect(exc, exc, {"rule": "fixed_indegree", "indegree": k_e},
+                 {"weight": w, "delay": d, "receptor": 0,
+                  "synapse_group": sg})
+    w, d = arrays((n - n_e) * k_e)
+    ngpu.Connect(exc, inh, {"rule": "fixed_indegree", "indegree": k_e},
+                 {"weight": w, "delay": d, "receptor": 0})
+    _, d = arrays(n * k_i)
+    ngpu.Connect(inh, neuron, {"rule": "fixed_indegree", "indegree": k_i},
+                 {"weight": w_in, "delay": d, "receptor": 1})
+    ngpu.Connect(pg, neuron, {"rule": "one_to_one"},
+                 {"weight": w_p, "delay": 0.1, "receptor": 0})
+    log("  connections in %.1f s" % (time.time() - t))
+    return neuron, n * k
+
+
 def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps):
     dt = ngpu.GetTimeResolution()
     ngpu.ActivateSpikeCount(neuron)
@@ -345,10 +387,11 @@ def main():
     ap.add_argument("--only-readme", action="store_true",
                     help="measure only the README network (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workloads", default="c2",
+    ap.add_argument("--workloads", default="c2,c3",
                     help="further configurations to measure after c1 "
-                         "(comma list of c2; empty for none)")
+                         "(comma list of c2, c3; empty for none)")
     ap.add_argument("--c2-scale", type=float, default=1.0)
+    ap.add_argument("--c3-scale", type=float, default=1.0)
     ap.add_argument("--cpu-neurons", type=int, default=100000)
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -444,6 +487,29 @@ def main():
             "rate_hz": round(r2["rate_hz"], 3),
             "kernels": r2["kernel_ms"],
             "roofline": roofline(r2)}
+        ngpu.ResetKernel()
+    if "c3" in args.workloads.split(","):
+        log("building c3 (%.3g x 2e5 aeif_cond_alpha neurons, STDP)"
+            % args.c3_scale)
+        neuron, n_syn = build_c3(ngpu, args.c3_scale)
+        r3 = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
+                     args.timing_steps)
+        log("c3:", json.dumps(r3))
+        out["c3"] = {
+            "workload": "c3: 2e5 aeif_cond_alpha (80 % E), fixed in-degree "
+                        "500, STDP on E->E (tau 20 ms, lambda 1e-3, alpha "
+                        "1.05, Wmax 0.1 nS, w0 ~ U(0.02, 0.06)), own 8 kHz "
+                        "Poisson per neuron (w 0.3 nS), RK5 tol 1e-3",
+            "neurons": r3["n_neurons"], "synapses": r3["n_synapses"],
+            "plastic_synapses": int(round(0.8 * r3["n_neurons"])) * 400,
+            "wall_s_per_bio_s": round(r3["wall_s_per_bio_s"], 4),
+            "ms_per_step": round(r3["ms_per_step"], 4),
+            "syn_events_per_s": round(r3["syn_events_per_s"], 1),
+            "rate_hz": round(r3["rate_hz"], 3),
+            "rk5_stages_per_neuron_step": round(
+                r3["rk5_stages_per_neuron_step"], 4),
+            "kernels": r3["kernel_ms"],
+            "roofline": roofline(r3)}
         ngpu.ResetKernel()
     if not args.no_cpu_baseline:
         log("cpu baseline")

@@ -387,9 +387,34 @@ __global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
   }
 }
 
-template <int P>
+// UNI: every parameter of the group is uniform (PField::ptr == nullptr),
+// so the values come straight from the kernel arguments
+template <int P, bool UNI>
 __device__ __forceinline__ void aeif_load_params(const AeifArgs &A, int i,
                                                  AeifLocal<P> &p) {
+  if (UNI) {
+    p.V_th = A.p.V_th.val;
+    p.Delta_T = A.p.Delta_T.val;
+    p.inv_Delta_T = A.p.inv_Delta_T.val;
+    p.g_L = A.p.g_L.val;
+    p.E_L = A.p.E_L.val;
+    p.inv_C_m = A.p.inv_C_m.val;
+    p.a = A.p.a.val;
+    p.b = A.p.b.val;
+    p.inv_tau_w = A.p.inv_tau_w.val;
+    p.I_e = A.p.I_e.val;
+    p.V_peak = A.p.V_peak.val;
+    p.V_reset = A.p.V_reset.val;
+    p.t_ref = A.p.t_ref.val;
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      p.E_rev[q] = A.p.E_rev[q].val;
+      p.inv_tau_rise[q] = A.p.inv_tau_rise[q].val;
+      p.inv_tau_decay[q] = A.p.inv_tau_decay[q].val;
+      p.g0[q] = A.p.g0[q].val;
+    }
+    return;
+  }
   p.V_th = pf(A.p.V_th, i);
   p.Delta_T = pf(A.p.Delta_T, i);
   p.inv_Delta_T = pf(A.p.inv_Delta_T, i);
@@ -528,7 +553,7 @@ __device__ __forceinline__ void aeif_count(const AeifArgs &A, int n_acc,
 // attempts.  Neurons that need more (those that spike or sit near
 // threshold, which would otherwise hold their 63 wave neighbours idle)
 // save their state and are listed per block for the second pass.
-template <int P>
+template <int P, bool UNI>
 __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
   constexpr int N = 2 + 2 * P;
   __shared__ uint32_t s_n, s_base;
@@ -546,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
   bool finished = false;
   if (active) {
     AeifLocal<P> p;
-    aeif_load_params<P>(A, i, p);
+    aeif_load_params<P, UNI>(A, i, p);
     float y[N], k1[N], inv_scal[N];
     y[0] = A.V[i];
     y[1] = A.w[i];
@@ -608,7 +633,7 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
 // entry as soon as the current neuron is done, so lanes stay busy while
 // neurons need different numbers of attempts, and waves beyond the list
 // end exit at once.
-template <int P>
+template <int P, bool UNI>
 __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
                                                 uint32_t vblock,
                                                 uint32_t vgrid) {
@@ -651,7 +676,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       i = (int)r.i;
       nsp = (int)r.nsp;
       n_att = (int)r.n_att;
-      aeif_load_params<P>(A, i, p);
+      aeif_load_params<P, UNI>(A, i, p);
       y[0] = A.V[i];
       y[1] = A.w[i];
 #pragma unroll
@@ -684,9 +709,9 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
   aeif_count<P>(A, n_acc, n_rej);
 }
 
-template <int P>
+template <int P, bool UNI>
 __global__ __launch_bounds__(kBlock) void aeif_pass2_kernel(AeifArgs A) {
-  aeif_pass2_body<P>(A, blockIdx.x, gridDim.x);
+  aeif_pass2_body<P, UNI>(A, blockIdx.x, gridDim.x);
 }
 
 template <bool PLASTIC>
@@ -700,12 +725,12 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
 // spikes of earlier steps and writes only the input accumulators), so the
 // latency-bound second pass and the atomic-bound delivery share the chip
 // without a second stream.
-template <int P>
+template <int P, bool UNI>
 __global__ __launch_bounds__(kBlock) void aeif_tail_kernel(AeifArgs A,
                                                            DeliveryArgs D,
                                                            uint32_t n2) {
   if (blockIdx.x < n2)
-    aeif_pass2_body<P>(A, blockIdx.x, n2);
+    aeif_pass2_body<P, UNI>(A, blockIdx.x, n2);
   else
     delivery_body<false>(D, blockIdx.x - n2, gridDim.x - n2);
 }
@@ -1070,7 +1095,12 @@ void launch_aeif_p(const AeifArgs &a, hipStream_t s) {
   int grid = (a.n + kBlock - 1) / kBlock;
   if (grid <= 0) return;
   if (a.do_step && a.strag) {
-    hipLaunchKernelGGL(aeif_pass1_kernel<P>, dim3(grid), dim3(kBlock), 0, s, a);
+    if (a.uniform)
+      hipLaunchKernelGGL((aeif_pass1_kernel<P, true>), dim3(grid), dim3(kBlock),
+                         0, s, a);
+    else
+      hipLaunchKernelGGL((aeif_pass1_kernel<P, false>), dim3(grid), dim3(kBlock),
+                         0, s, a);
   } else {
     hipLaunchKernelGGL(aeif_kernel<P>, dim3(grid), dim3(kBlock), 0, s, a);
   }
@@ -1080,11 +1110,18 @@ template <int P>
 void launch_aeif_pass2_p(const AeifArgs &a, const DeliveryArgs *d,
                          int d_grid, hipStream_t s) {
   int grid2 = a.pass2_grid > 0 ? a.pass2_grid : 256;
-  if (d)
-    hipLaunchKernelGGL(aeif_tail_kernel<P>, dim3(grid2 + d_grid), dim3(kBlock),
-                       0, s, a, *d, (uint32_t)grid2);
+  if (d && a.uniform)
+    hipLaunchKernelGGL((aeif_tail_kernel<P, true>), dim3(grid2 + d_grid),
+                       dim3(kBlock), 0, s, a, *d, (uint32_t)grid2);
+  else if (d)
+    hipLaunchKernelGGL((aeif_tail_kernel<P, false>), dim3(grid2 + d_grid),
+                       dim3(kBlock), 0, s, a, *d, (uint32_t)grid2);
+  else if (a.uniform)
+    hipLaunchKernelGGL((aeif_pass2_kernel<P, true>), dim3(grid2), dim3(kBlock),
+                       0, s, a);
   else
-    hipLaunchKernelGGL(aeif_pass2_kernel<P>, dim3(grid2), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((aeif_pass2_kernel<P, false>), dim3(grid2), dim3(kBlock),
+                       0, s, a);
 }
 
 }  // namespace

@@ -107,6 +107,7 @@ struct AeifArgs {
   unsigned long long strag_cap;
   int budget;
   int pass2_grid;
+  int uniform;                // every parameter PField of the group uniform
 };
 
 struct IafArgs {

@@ -169,3 +169,62 @@ def n_synapses(spec, exclude_generators=True):
             continue
         tot += src.size
     return tot
+
+
+# ---------------------------------------------------------------------
+# c3: AdEx network with pair-based STDP on the E -> E synapses
+# ---------------------------------------------------------------------
+C3_STDP = {"tau_plus": 20.0, "tau_minus": 20.0, "lambda": 1.0e-3,
+           "alpha": 1.05, "mu_plus": 1.0, "mu_minus": 1.0, "Wmax": 0.1}
+C3_NEURON = {"C_m": 281.0, "g_L": 30.0, "E_L": -70.6, "V_th": -50.4,
+             "Delta_T": 2.0, "a": 4.0, "tau_w": 144.0, "b": 80.5,
+             "V_reset": -60.0, "t_ref": 0.0, "E_rev": [0.0, -85.0],
+             "tau_syn": [2.0, 10.0]}
+
+
+def c3_spec(scale=1.0, seed=4321, k=500, w_in=0.25, w_p=0.3, rate=8000.0,
+            tol=1e-3, dt=0.1):
+    """2e5 x scale aeif_cond_alpha neurons (80 % E), 2 ports (E_rev 0 /
+    -85 mV, tau_syn 2 / 10 ms), fixed in-degree k (80 % from E, 20 % from
+    I, without replacement).  E synapses start at w ~ U(0.02, 0.06) nS;
+    those onto E neurons are plastic (pair-based STDP, tau+- 20 ms,
+    lambda 1e-3, alpha 1.05, Wmax 0.1 nS, mu+- 1).  I synapses w_in nS on
+    port 1.  Delays uniform 1-20 steps.  Each neuron has its own Poisson
+    generator at `rate` Hz with weight w_p nS."""
+    rng = np.random.default_rng(seed)
+    n = max(10, int(round(200000 * scale)))
+    n_e = int(round(0.8 * n))
+    k_e, k_i = int(round(0.8 * k)), k - int(round(0.8 * k))
+    src_e = np.concatenate([rng.choice(n_e, k_e, replace=False)
+                            for _ in range(n)]).astype(np.int32)
+    tgt_e = np.repeat(np.arange(n, dtype=np.int32), k_e)
+    src_i = np.concatenate([n_e + rng.choice(n - n_e, k_i, replace=False)
+                            for _ in range(n)]).astype(np.int32)
+    tgt_i = np.repeat(np.arange(n, dtype=np.int32), k_i)
+    w_e = rng.uniform(0.02, 0.06, src_e.size).astype(np.float32)
+    d_e = (rng.integers(1, 21, src_e.size) * dt).astype(np.float32)
+    d_i = (rng.integers(1, 21, src_i.size) * dt).astype(np.float32)
+    ee = tgt_e < n_e
+    v0 = rng.uniform(-70.0, -55.0, n).astype(np.float32)
+    gen0 = n
+    return {
+        "dt": dt, "seed": seed,
+        "groups": [{"model": "aeif_cond_alpha", "n": n, "n_port": 2,
+                    "params": dict(C3_NEURON), "vars": {"V_m": v0},
+                    "rk5": {"rk5_tol": tol}},
+                   {"model": "poisson_generator", "n": n}],
+        "stdp": [dict(C3_STDP)],
+        "poisson": [{"node": gen0 + i, "rate": rate} for i in range(n)],
+        "conns": [
+            {"src": src_e[ee], "tgt": tgt_e[ee], "weight": w_e[ee],
+             "delay": d_e[ee], "port": 0, "syn_group": 1},
+            {"src": src_e[~ee], "tgt": tgt_e[~ee], "weight": w_e[~ee],
+             "delay": d_e[~ee], "port": 0},
+            {"src": src_i, "tgt": tgt_i,
+             "weight": np.float32(w_in), "delay": d_i, "port": 1},
+            {"src": np.arange(gen0, gen0 + n, dtype=np.int32),
+             "tgt": np.arange(n, dtype=np.int32), "weight": np.float32(w_p),
+             "delay": np.float32(dt), "port": 0},
+        ],
+        "populations": [(0, n_e), (n_e, n)],
+    }

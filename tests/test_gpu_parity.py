@@ -274,3 +274,39 @@ def test_c2_iaf_network_bit_exact(ngpu):
     log("spikes engine", len(eng), "oracle", len(orc), net.counters())
     assert len(orc) > 10000
     assert eng == orc
+
+
+def test_c3_stdp_network_statistics(ngpu):
+    """c3-shaped plastic network at 2 % size (4000 aeif_cond_alpha neurons,
+    2e6 synapses, STDP on E -> E), 500 ms: population rates within 2 % of
+    the FP64 oracle; the final E -> E weights stay in [0, Wmax] and their
+    mean and standard deviation agree with the oracle's to 0.5 %."""
+    from neurongpu_amd import workloads as wl
+    log("case c3_scaled")
+    spec = wl.c3_spec(scale=0.02)
+    n = spec["groups"][0]["n"]
+    ne = spec["populations"][0][1]
+    groups, _ = nb.build_engine(ngpu, spec)
+    ngpu.ActivateSpikeCount(groups[0])
+    ngpu.Simulate(500.0)
+    cnt = np.array(ngpu.GetNeuronIntVar(0, n, "spike_count"), dtype=float)
+    st = ngpu.GetStatus(ngpu.GetConnections(ngpu.NodeSeq(0, ne),
+                                            ngpu.NodeSeq(0, ne)))
+    wg = np.array([d["weight"] for d in st])
+    net, _ = nb.build_oracle(spec, "f64")
+    net.run(5000, 16)
+    s_, nd, ct = net.spikes()
+    src, tgt, port, d, wo = net.connections()
+    wo = wo[(src < ne) & (tgt < ne)]
+    for name, (a, b) in zip(("E", "I"), spec["populations"]):
+        r_g = cnt[a:b].sum() / (b - a) / 0.5
+        m = (nd >= a) & (nd < b)
+        r_o = ct[m].sum() / (b - a) / 0.5
+        log(name, "rate gpu", r_g, "fp64 oracle", r_o)
+        assert abs(r_g - r_o) <= 0.02 * r_o
+    log("EE weights gpu mean %.6f std %.6f, oracle mean %.6f std %.6f"
+        % (wg.mean(), wg.std(), wo.mean(), wo.std()))
+    assert wg.size == wo.size
+    assert wg.min() >= 0.0 and wg.max() <= 0.1 + 1e-7
+    assert abs(wg.mean() - wo.mean()) <= 0.005 * wo.mean()
+    assert abs(wg.std() - wo.std()) <= 0.005 * wo.std()

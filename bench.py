@@ -220,7 +220,8 @@ def build_c3(ngpu, scale=1.0, seed=4321, k=500, w_in=0.25, w_p=0.3,
     return neuron, n * k
 
 
-def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps):
+def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps,
+            after_warmup=None):
     dt = ngpu.GetTimeResolution()
     ngpu.ActivateSpikeCount(neuron)
     t = time.time()
@@ -228,6 +229,7 @@ def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps):
     log("  calibrate (CSR build + upload) in %.1f s" % (time.time() - t))
     if warmup > 0:
         ngpu.Simulate(warmup * dt)
+    extra = after_warmup() if after_warmup else None
     s0 = ngpu.GetHotPathStats()
     c0 = int(np.sum(ngpu.GetNeuronIntVar(neuron.i0, neuron.n,
                                          "spike_count")))
@@ -286,6 +288,8 @@ def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps):
         / 1e9}
     res["n_neurons"] = neuron.n
     res["n_synapses"] = n_syn
+    if extra is not None:
+        res["after_warmup"] = extra
     return res
 
 
@@ -387,9 +391,10 @@ def main():
     ap.add_argument("--only-readme", action="store_true",
                     help="measure only the README network (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workloads", default="c2,c3",
+    ap.add_argument("--workloads", default="c2,c3,c4",
                     help="further configurations to measure after c1 "
-                         "(comma list of c2, c3; empty for none)")
+                         "(comma list of c2, c3, c4; empty for none)")
+    ap.add_argument("--c4-instances", type=int, default=64)
     ap.add_argument("--c2-scale", type=float, default=1.0)
     ap.add_argument("--c3-scale", type=float, default=1.0)
     ap.add_argument("--cpu-neurons", type=int, default=100000)
@@ -510,6 +515,43 @@ def main():
                 r3["rk5_stages_per_neuron_step"], 4),
             "kernels": r3["kernel_ms"],
             "roofline": roofline(r3)}
+        ngpu.ResetKernel()
+    if "c4" in args.workloads.split(","):
+        from neurongpu_amd import workloads as wl
+        from neurongpu_amd.autotune import autotune
+        inst = wl.c4_instances()
+        if args.c4_instances < len(inst):
+            inst = inst[:args.c4_instances]
+        log("building c4 (%d AdEx networks, %d neurons)"
+            % (len(inst), sum(x["n"] for x in inst)))
+        t = time.time()
+        neuron, ranges, n_syn = wl.build_ensemble(ngpu, inst)
+        log("  connections in %.1f s" % (time.time() - t))
+        r4 = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
+                     args.timing_steps,
+                     after_warmup=lambda: autotune(ngpu, steps=40, log=log))
+        tuned = r4["after_warmup"]
+        log("  autotuned:", tuned)
+        log("c4:", json.dumps(r4))
+        out["c4"] = {
+            "workload": "c4: %d independent aeif_cond_beta networks in one "
+                        "node group (sizes log-U[5e3, 5e4], in-degree "
+                        "U{100..1000}, weight scale U(0.5, 2) x (0.05, "
+                        "0.25) nS, Poisson U(4, 12) kHz w 0.1), one launch "
+                        "per kernel per step, RK5 tol 1e-3" % len(inst),
+            "instances": len(inst),
+            "neurons": r4["n_neurons"], "synapses": r4["n_synapses"],
+            "autotuned": tuned,
+            "wall_s_per_bio_s": round(r4["wall_s_per_bio_s"], 4),
+            "wall_s_per_bio_s_per_instance": round(
+                r4["wall_s_per_bio_s"] / len(inst), 5),
+            "ms_per_step": round(r4["ms_per_step"], 4),
+            "syn_events_per_s": round(r4["syn_events_per_s"], 1),
+            "rate_hz": round(r4["rate_hz"], 3),
+            "rk5_stages_per_neuron_step": round(
+                r4["rk5_stages_per_neuron_step"], 4),
+            "kernels": r4["kernel_ms"],
+            "roofline": roofline(r4)}
         ngpu.ResetKernel()
     if not args.no_cpu_baseline:
         log("cpu baseline")

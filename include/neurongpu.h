@@ -476,6 +476,16 @@ class NeuronGPU {
   // generator, 4 delivery, 5 post-synaptic STDP, 6 record)
   int SetKernelTiming(int on);
   int GetKernelTimes(double *ms8, long long *cnt8);
+  // no reference counterpart: learned RK5 step-size proposer
+  int ProposerLog(int on, long long max_samples);
+  long long ProposerNSamples();
+  int ProposerGetSamples(float *out, long long n);
+  int ProposerSetSamples(const float *in, long long n);
+  float ProposerTrain(int n_iter, int batch, float lr, int adam,
+                      int full_batch);
+  int ProposerGetWeights(float *w);
+  int ProposerSetWeights(const float *w);
+  int ProposerUse(int on);
 };
 
 #endif

@@ -192,10 +192,33 @@ extern "C" {
 
   /* Per-kernel device time measured with HIP events on the engine's stream
    * (on != 0 enables; enabling or disabling clears the totals).  Kinds:
-   * 0 AdEx RK5 update, 1 iaf_psc_exp update, 2 spike detector / parrot,
-   * 3 spike generator, 4 delivery, 5 post-synaptic STDP, 6 record. */
+   * 0 AdEx RK5 update (first pass), 1 iaf_psc_exp update, 2 spike
+   * detector / parrot, 3 spike generator, 4 delivery, 5 post-synaptic STDP,
+   * 6 record, 7 second RK5 pass (with the delivery when they share a
+   * launch). */
   int NeuronGPU_SetKernelTiming(int on);
   int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8);
+
+  /* Learned RK5 step-size proposer (6-32-32-1 MLP, BF16 inference, FP32
+   * training), used for the RK5 steps of the second pass of the AdEx update
+   * when enabled; the RK5 error test still accepts or rejects every
+   * attempt.  Samples are 8 floats: 6 features, target, unused. */
+  /* on != 0: log one sample per second-pass RK5 attempt, up to max_samples
+   * (the buffer is kept until the next call with on != 0) */
+  int NeuronGPU_ProposerLog(int on, long long max_samples);
+  long long NeuronGPU_ProposerNSamples();
+  int NeuronGPU_ProposerGetSamples(float *out, long long n);
+  int NeuronGPU_ProposerSetSamples(float *in, long long n);
+  /* n_iter minibatch steps of `batch` samples (full_batch != 0: samples
+   * 0..batch-1 every step), SGD (adam == 0) or Adam with step size lr;
+   * *loss receives the mean squared error of the last minibatch */
+  int NeuronGPU_ProposerTrain(int n_iter, int batch, float lr, int adam,
+                              int full_batch, float *loss);
+  int NeuronGPU_ProposerGetWeights(float *w1313);
+  int NeuronGPU_ProposerSetWeights(float *w1313);
+  /* on != 0: the second RK5 pass starts its steps from the proposer's step
+   * size instead of the plain controller's */
+  int NeuronGPU_ProposerUse(int on);
 
 #ifdef __cplusplus
 }

@@ -190,6 +190,17 @@ class Engine {
   };
   std::vector<DeferredPass2> deferred_pass2;
   bool fuse_delivery = true;  // delivery in the last second-pass launch
+  // learned RK5 step-size proposer
+  DBuf<float> d_prop_w, d_prop_m, d_prop_v, d_prop_grad, d_prop_loss;
+  DBuf<uint16_t> d_prop_bf16;
+  DBuf<float> d_prop_log;
+  DBuf<unsigned long long> d_prop_log_n;
+  unsigned long long prop_log_max = 0;
+  bool prop_log_on = false, prop_use = false;
+  uint32_t prop_iter = 0;
+  void PropInit();
+  int delivery_grid = 2048;   // blocks of the delivery kernel
+  int delivery_tile_max = 64; // largest pairs per wave tile
   int max_port = 1;
   int R = 0;
   long long ring_cap = 0;

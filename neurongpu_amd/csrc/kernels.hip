@@ -26,6 +26,7 @@
 
 #include "ngpu_device.h"
 #include "ngpu_kernels.h"
+#include "proposer.h"
 
 namespace ngpu {
 
@@ -442,14 +443,65 @@ __device__ __forceinline__ void aeif_load_params(const AeifArgs &A, int i,
 // is in the group arrays; h_begin is the step size the current RK5 step
 // started with, from which the second pass recomputes the step's first
 // stage and error scale exactly.
+// Learned step-size proposer (inference): features of a neuron at the
+// start of an RK5 step and the MLP with BF16 weights, FP32 accumulation.
+template <int P>
+__device__ __forceinline__ void prop_features(const AeifLocal<P> &p,
+                                              const float (&y)[2 + 2 * P],
+                                              float h, float err_last,
+                                              float inv_dt,
+                                              float (&f)[kPropIn]) {
+  f[0] = (y[0] - p.E_L) * 0.1f;
+  f[1] = y[1] * 0.01f;
+  f[2] = y[2];
+  float gi = 0.0f;
+#pragma unroll
+  for (int q = 1; q < P; q++) gi += y[2 + 2 * q];
+  f[3] = gi;
+  f[4] = __log2f(h * inv_dt);
+  float e = err_last > 1e-8f ? err_last : 1e-8f;
+  f[5] = __log2f(e) * (0.30103f * 0.25f);
+}
+
+__device__ __forceinline__ float bf16f(uint16_t u) {
+  return __uint_as_float((uint32_t)u << 16);
+}
+
+__device__ __forceinline__ float prop_mlp(const uint16_t *__restrict__ w,
+                                          const float (&x)[kPropIn]) {
+  float h1[kPropHid];
+#pragma unroll
+  for (int j = 0; j < kPropHid; j++) {
+    float a = bf16f(w[kPropB1 + j]);
+#pragma unroll
+    for (int i = 0; i < kPropIn; i++)
+      a = __builtin_fmaf(bf16f(w[kPropW1 + j * kPropIn + i]), x[i], a);
+    h1[j] = a > 0.0f ? a : 0.0f;
+  }
+  float out = bf16f(w[kPropB3]);
+#pragma unroll
+  for (int j = 0; j < kPropHid; j++) {
+    float a = bf16f(w[kPropB2 + j]);
+#pragma unroll
+    for (int i = 0; i < kPropHid; i++)
+      a = __builtin_fmaf(bf16f(w[kPropW2 + j * kPropHid + i]), h1[i], a);
+    a = a > 0.0f ? a : 0.0f;
+    out = __builtin_fmaf(bf16f(w[kPropW3 + j]), a, out);
+  }
+  return out;
+}
+
 constexpr int kStragLists = 64;  // sub-lists of resume records
 
 struct StragRec {
   uint32_t i;
   uint32_t nsp;
   uint32_t n_att;
+  uint32_t flags;     // bit 0: saved at an RK5 step boundary
   float h_begin;
+  float err_last;     // scaled error of the neuron's last attempt
 };
+static_assert(sizeof(StragRec) == kStragRecBytes, "resume record size");
 
 // One RK5 attempt of a neuron plus the step-boundary logic of RK5Update
 // and ExternalUpdate (src/rk5.h:70-162, src/aeif_cond_beta_kernel.h:
@@ -461,7 +513,7 @@ __device__ __forceinline__ bool aeif_attempt_and_update(
     const AeifArgs &A, const AeifLocal<P> &p, float (&y)[2 + 2 * P],
     const float (&k1)[2 + 2 * P], const float (&inv_scal)[2 + 2 * P],
     double &x, float &h, float &refr, int &nsp, int &n_att, bool &need_k1,
-    int &n_acc, int &n_rej) {
+    int &n_acc, int &n_rej, float &err_out, float &hh_out) {
   constexpr int N = 2 + 2 * P;
   const double x1 = A.x1;
   float yn[N];
@@ -472,6 +524,8 @@ __device__ __forceinline__ bool aeif_attempt_and_update(
   h = hh;
   float err = aeif_rk5_attempt<P>(p, refr, y, k1, inv_scal, hh, yn);
   err = err / A.eps;
+  err_out = err;
+  hh_out = hh;
   n_att++;
   if (err <= 1.0f || (double)hh <= A.h_min_acc) {
     n_acc++;
@@ -590,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
     float refr = A.refr[i];
     int n_att = 0;
     bool need_k1 = true;
-    float h_begin = h;
+    float h_begin = h, err_last = 1.0f, hh_used = h;
     for (;;) {
       if (n_att >= A.budget) break;
       if (need_k1) {
@@ -599,7 +653,8 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
         need_k1 = false;
       }
       if (aeif_attempt_and_update<P>(A, p, y, k1, inv_scal, x, h, refr, nsp,
-                                     n_att, need_k1, n_acc, n_rej)) {
+                                     n_att, need_k1, n_acc, n_rej, err_last,
+                                     hh_used)) {
         finished = true;
         break;
       }
@@ -610,7 +665,9 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
       rec.i = (uint32_t)i;
       rec.nsp = (uint32_t)nsp;
       rec.n_att = (uint32_t)n_att;
+      rec.flags = need_k1 ? 1u : 0u;
       rec.h_begin = need_k1 ? h : h_begin;
+      rec.err_last = err_last;
     }
   }
   emit_spikes(A.out, active && finished, ii, node, nsp, (float)nsp);
@@ -662,6 +719,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
   float h = 0.0f, refr = 0.0f;
   int nsp = 0, n_att = 0, n_acc = 0, n_rej = 0;
   int i = 0;
+  float err_last = 1.0f;
   bool need_k1 = false, have = false;
   for (;;) {
     if (!have && j < total) {
@@ -676,6 +734,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       i = (int)r.i;
       nsp = (int)r.nsp;
       n_att = (int)r.n_att;
+      err_last = r.err_last;
       aeif_load_params<P, UNI>(A, i, p);
       y[0] = A.V[i];
       y[1] = A.w[i];
@@ -687,20 +746,58 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       x = A.x[i];
       h = A.h[i];
       refr = A.refr[i];
-      aeif_rk5_begin<P>(p, refr, y, r.h_begin, k1, inv_scal);
-      need_k1 = false;
+      if (r.flags & 1u) {
+        need_k1 = true;  // saved at a step boundary: a new RK5 step starts
+      } else {
+        aeif_rk5_begin<P>(p, refr, y, r.h_begin, k1, inv_scal);
+        need_k1 = false;
+      }
       have = true;
     }
     if (__ballot(have) == 0ull) break;
-    bool finished = false;
+    bool finished = false, attempted = false;
+    float feat[kPropIn];
+    float err = 1.0f, hh = 0.0f;
     if (have) {
       if (need_k1) {
+        if (A.prop_w) {
+          prop_features<P>(p, y, h, err_last, 1.0f / A.dt, feat);
+          float o = prop_mlp(A.prop_w, feat);
+          float hp = A.dt * exp2f(o) * kPropSafety;
+          h = hp > A.h_min ? hp : A.h_min;
+        }
         aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
         need_k1 = false;
       }
+      if (A.prop_log) prop_features<P>(p, y, h, err_last, 1.0f / A.dt, feat);
       finished = aeif_attempt_and_update<P>(A, p, y, k1, inv_scal, x, h, refr,
-                                            nsp, n_att, need_k1, n_acc, n_rej);
+                                            nsp, n_att, need_k1, n_acc, n_rej,
+                                            err, hh);
+      err_last = err;
+      attempted = true;
       if (finished) aeif_store_state<P>(A, i, y, x, h, refr);
+    }
+    if (A.prop_log) {
+      // sample: features before the attempt, target log2 of the step that
+      // would have given a scaled error of 1
+      unsigned long long m = __ballot(attempted);
+      if (m != 0ull) {
+        uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+        unsigned long long base = 0;
+        if (lane_id() == leader)
+          base = atomicAdd(A.prop_log_n, (unsigned long long)__popcll(m));
+        base = __shfl(base, (int)leader);
+        unsigned long long slot2 = base + lanes_below(m);
+        if (attempted && slot2 < A.prop_log_max) {
+          float e = err > CK::err_min ? err : CK::err_min;
+          float t = __log2f(hh / A.dt) - 0.2f * __log2f(e);
+          float *o = A.prop_log + slot2 * kPropSample;
+#pragma unroll
+          for (int k = 0; k < kPropIn; k++) o[k] = feat[k];
+          o[kPropIn] = t;
+          o[kPropIn + 1] = 0.0f;
+        }
+      }
     }
     emit_spikes(A.out, finished, finished ? i : 0, A.i0 + (finished ? i : 0),
                 nsp, (float)nsp);
@@ -900,10 +997,11 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
   const uint32_t lane = lane_id();
   const uint32_t wave = (uint32_t)(vblock * (kBlock / 64) + tid / 64);
   const uint32_t n_waves = vgrid * (kBlock / 64);
-  // pairs per wave tile: the largest power of two in [1, 64] that still
-  // gives every wave of the grid a tile, so that steps with few spikes
-  // spread their synapses over many waves instead of a few long loops
-  uint32_t pt = 64;
+  // pairs per wave tile: the largest power of two in [1, tile_max] that
+  // still gives every wave of the grid a tile, so that steps with few
+  // spikes spread their synapses over many waves instead of a few long
+  // loops
+  uint32_t pt = (uint32_t)A.tile_max;
   while (pt > 1 && (total + pt - 1) / pt < n_waves) pt >>= 1;
   const uint32_t n_tiles = (total + pt - 1) / pt;
   unsigned long long n_events = 0;

@@ -454,4 +454,24 @@ int NeuronGPU_SetKernelTiming(int on)
     RET(int, 0, ret = instance->SetKernelTiming(on))
 int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8)
     RET(int, 0, ret = instance->GetKernelTimes(ms8, cnt8))
+int NeuronGPU_ProposerLog(int on, long long max_samples)
+    RET(int, 0, ret = instance->ProposerLog(on, max_samples))
+long long NeuronGPU_ProposerNSamples()
+    RET(long long, 0, ret = instance->ProposerNSamples())
+int NeuronGPU_ProposerGetSamples(float *out, long long n)
+    RET(int, 0, ret = instance->ProposerGetSamples(out, n))
+int NeuronGPU_ProposerSetSamples(float *in, long long n)
+    RET(int, 0, ret = instance->ProposerSetSamples(in, n))
+int NeuronGPU_ProposerTrain(int n_iter, int batch, float lr, int adam,
+                            int full_batch, float *loss)
+    RET(int, 0, {
+      float l = instance->ProposerTrain(n_iter, batch, lr, adam, full_batch);
+      if (loss) *loss = l;
+    })
+int NeuronGPU_ProposerGetWeights(float *w1313)
+    RET(int, 0, ret = instance->ProposerGetWeights(w1313))
+int NeuronGPU_ProposerSetWeights(float *w1313)
+    RET(int, 0, ret = instance->ProposerSetWeights(w1313))
+int NeuronGPU_ProposerUse(int on)
+    RET(int, 0, ret = instance->ProposerUse(on))
 }

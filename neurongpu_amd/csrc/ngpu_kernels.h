@@ -25,6 +25,7 @@ constexpr long long kLastSpikeTimeGuard = 0x70000000LL;
 constexpr int kMaxAttemptsPerStep = 100000;
 constexpr int kCounterSlots = 1024;   // spread statistic counters over L2
 constexpr uint32_t kPoissonTag = 0x504F4953u;
+constexpr int kStragRecBytes = 24;    // resume record of the two-pass update
 
 // A parameter that is either uniform over a group (val) or per neuron (ptr).
 struct PField {
@@ -108,6 +109,13 @@ struct AeifArgs {
   int budget;
   int pass2_grid;
   int uniform;                // every parameter PField of the group uniform
+  // learned step-size proposer, second pass only: BF16 weights (nullptr:
+  // plain controller) and the sample log of (features, target) per attempt
+  // (nullptr: no logging)
+  const uint16_t *prop_w;
+  float *prop_log;
+  unsigned long long *prop_log_n;
+  unsigned long long prop_log_max;
 };
 
 struct IafArgs {
@@ -152,6 +160,8 @@ struct DeliveryArgs {
   int R;
   long long gs;                 // global step index of this step
   int D;                        // number of delay slots (max d_int + 1)
+  int tile_max;                 // largest (spike, delay) pairs per wave tile,
+                                // a power of two in [1, 64]
   const unsigned long long *base;  // [n_total+1]
   const uint32_t *seg;          // [n_total][D+1]
   unsigned long long *rec;      // synapse records, weight in high word

@@ -1,0 +1,58 @@
+// Learned RK5 step-size proposer: a 6 -> 32 -> 32 -> 1 ReLU MLP that
+// predicts, from a neuron's state at the start of an RK5 step, the largest
+// step the embedded error estimate would accept.  The RK5 error test still
+// accepts or rejects every attempt, so accuracy is that of the plain
+// controller; only the number of attempts changes.
+//
+// It has no counterpart in the reference.  Used only in the second pass of
+// the two-pass AdEx update (the neurons that need more than one attempt in
+// a time step).
+#pragma once
+#include <stdint.h>
+
+namespace ngpu {
+
+constexpr int kPropIn = 6;
+constexpr int kPropHid = 32;
+// parameter layout (FP32 master copy and BF16 inference copy)
+constexpr int kPropW1 = 0;                          // [32][6]
+constexpr int kPropB1 = kPropW1 + kPropHid * kPropIn;   // [32]
+constexpr int kPropW2 = kPropB1 + kPropHid;             // [32][32]
+constexpr int kPropB2 = kPropW2 + kPropHid * kPropHid;  // [32]
+constexpr int kPropW3 = kPropB2 + kPropHid;             // [32]
+constexpr int kPropB3 = kPropW3 + kPropHid;             // [1]
+constexpr int kPropNParam = kPropB3 + 1;                // 1313
+constexpr int kPropSample = 8;   // floats per logged sample: 6 features,
+                                 // target, spare
+constexpr float kPropSafety = 0.9f;
+
+// Features of a neuron at the start of an RK5 step, from quantities the
+// plain controller has at hand:
+//   (V - E_L) / 10 mV, w / 100 pA, g_exc (nS), g_inh (nS, sum of ports >= 1),
+//   log2(h / dt) of the step the controller would take, and
+//   log10(err) / 4 of the last attempt's scaled error.
+// Target of a sample (one RK5 attempt of size hh with scaled error err):
+//   log2(hh * err^(-1/5) / dt), the step that would give err = 1.
+
+// Training on logged samples (proposer.hip): minibatches of `batch` samples
+// drawn with Philox from [0, n), mean squared error, SGD or Adam.
+struct PropTrainArgs {
+  const float *samples;        // [n][kPropSample]
+  unsigned long long n;
+  float *w;                    // FP32 master weights
+  float *grad;                 // [kPropNParam] accumulated gradient
+  float *loss;                 // [1] accumulated loss
+  float *m, *v;                // Adam moments
+  uint16_t *w_bf16;            // inference copy
+  int batch;
+  uint32_t iter;               // Philox counter of the minibatch
+  uint32_t seed;
+  float lr;
+  int adam;
+  int full_batch;              // 1: samples 0..batch-1 in order
+};
+
+const char *LaunchPropTrainStep(const PropTrainArgs &a, void *stream);
+void LaunchPropToBf16(const float *w, uint16_t *w_bf16, void *stream);
+
+}  // namespace ngpu

@@ -188,6 +188,15 @@ _SIG = {
     "SetKernelTiming": (c_int, (c_int,)),
     "GetKernelTimes": (c_int, (ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_longlong))),
+    "ProposerLog": (c_int, (c_int, ctypes.c_longlong)),
+    "ProposerNSamples": (ctypes.c_longlong, ()),
+    "ProposerGetSamples": (c_int, (c_void_p, ctypes.c_longlong)),
+    "ProposerSetSamples": (c_int, (c_void_p, ctypes.c_longlong)),
+    "ProposerTrain": (c_int, (c_int, c_int, c_float, c_int, c_int,
+                              ctypes.POINTER(c_float))),
+    "ProposerGetWeights": (c_int, (c_void_p,)),
+    "ProposerSetWeights": (c_int, (c_void_p,)),
+    "ProposerUse": (c_int, (c_int,)),
 }
 
 _C = {}
@@ -1398,3 +1407,60 @@ def GetKernelTimes():
     cnt = (ctypes.c_longlong * 8)()
     _call("GetKernelTimes", ms, cnt)
     return {KERNEL_KINDS[k]: (ms[k], cnt[k]) for k in range(8) if cnt[k]}
+
+
+# ---------------------------------------------------------------------
+# learned RK5 step-size proposer (no counterpart in the reference)
+PROPOSER_N_PARAM = 1313
+PROPOSER_SAMPLE = 8
+
+
+def ProposerLog(on, max_samples=10000000):
+    "Log (features, target) samples of second-pass RK5 attempts"
+    return _call("ProposerLog", 1 if on else 0, int(max_samples))
+
+
+def ProposerNSamples():
+    return _call("ProposerNSamples")
+
+
+def ProposerGetSamples(n=None):
+    if n is None:
+        n = ProposerNSamples()
+    out = np.zeros((n, PROPOSER_SAMPLE), dtype=np.float32)
+    if n:
+        _call("ProposerGetSamples", out.ctypes.data_as(c_void_p), n)
+    return out
+
+
+def ProposerSetSamples(samples):
+    a = np.ascontiguousarray(samples, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != PROPOSER_SAMPLE:
+        raise ValueError("samples must have shape (n, %d)" % PROPOSER_SAMPLE)
+    return _call("ProposerSetSamples", a.ctypes.data_as(c_void_p), a.shape[0])
+
+
+def ProposerTrain(n_iter, batch=8192, lr=1e-3, adam=True, full_batch=False):
+    "Train the proposer; returns the mean squared error of the last batch"
+    loss = c_float(0.0)
+    _call("ProposerTrain", int(n_iter), int(batch), float(lr),
+          1 if adam else 0, 1 if full_batch else 0, ctypes.byref(loss))
+    return loss.value
+
+
+def ProposerGetWeights():
+    w = np.zeros(PROPOSER_N_PARAM, dtype=np.float32)
+    _call("ProposerGetWeights", w.ctypes.data_as(c_void_p))
+    return w
+
+
+def ProposerSetWeights(w):
+    a = np.ascontiguousarray(w, dtype=np.float32)
+    if a.size != PROPOSER_N_PARAM:
+        raise ValueError("expected %d weights" % PROPOSER_N_PARAM)
+    return _call("ProposerSetWeights", a.ctypes.data_as(c_void_p))
+
+
+def ProposerUse(on):
+    "Start second-pass RK5 steps from the proposer's step size"
+    return _call("ProposerUse", 1 if on else 0)

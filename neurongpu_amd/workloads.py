@@ -228,3 +228,70 @@ def c3_spec(scale=1.0, seed=4321, k=500, w_in=0.25, w_p=0.3, rate=8000.0,
         ],
         "populations": [(0, n_e), (n_e, n)],
     }
+
+
+# ---------------------------------------------------------------------
+# c4: batched ensemble of independent AdEx networks
+# ---------------------------------------------------------------------
+C4_NEURON = {"C_m": 281.0, "g_L": 30.0, "E_L": -70.6, "V_th": -50.4,
+             "Delta_T": 2.0, "a": 4.0, "tau_w": 144.0, "b": 80.5,
+             "V_reset": -60.0, "t_ref": 0.0, "E_rev": [0.0, -85.0],
+             "tau_rise": [0.2, 0.5], "tau_decay": [2.0, 10.0]}
+
+
+def c4_instances(n_inst=64, seed=6464, n_lo=5000, n_hi=50000):
+    """Per-instance draws of the ensemble configuration: size log-uniform
+    in [n_lo, n_hi], in-degree uniform in {100..1000}, E/I weight scale
+    U(0.5, 2) times (0.05, 0.25) nS, Poisson rate U(4, 12) kHz."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_inst):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        k = int(rng.integers(100, 1001))
+        sc = float(rng.uniform(0.5, 2.0))
+        rate = float(rng.uniform(4000.0, 12000.0))
+        out.append({"n": n, "k": k, "w_e": 0.05 * sc, "w_i": 0.25 * sc,
+                    "rate": rate})
+    return out
+
+
+def build_ensemble(ngpu, instances, seed=6464, tol=1e-3, w_p=0.1):
+    """All instances in one aeif_cond_beta node group (one launch per
+    kernel per step for the whole ensemble): instance k occupies a
+    contiguous node range, draws 80 % of its in-degree from its own E
+    neurons (port 0) and 20 % from its own I neurons (port 1), delays
+    1-20 steps, and has its own Poisson generator.  Returns the node
+    sequence of the neurons, the instance ranges and the synapse count."""
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetRandomSeed(seed)
+    rng = np.random.default_rng(seed)
+    n_tot = sum(x["n"] for x in instances)
+    neuron = ngpu.Create("aeif_cond_beta", n_tot, 2)
+    ngpu.SetStatus(neuron, dict(C4_NEURON))
+    ngpu.SetStatus(neuron, {"V_m": {"array": rng.uniform(
+        -70.0, -55.0, n_tot).astype(np.float32)}, "rk5_tol": tol})
+    pg = ngpu.Create("poisson_generator", len(instances))
+    ranges, i0, n_syn = [], 0, 0
+    for j, x in enumerate(instances):
+        n, k = x["n"], x["k"]
+        n_e = int(round(0.8 * n))
+        k_e = min(int(round(0.8 * k)), n_e)
+        k_i = min(k - int(round(0.8 * k)), n - n_e)
+        net = neuron[i0:i0 + n]
+        exc, inh = neuron[i0:i0 + n_e], neuron[i0 + n_e:i0 + n]
+        d_e = (rng.integers(1, 21, n * k_e) * 0.1).astype(np.float32)
+        ngpu.Connect(exc, net, {"rule": "fixed_indegree", "indegree": k_e},
+                     {"weight": x["w_e"], "delay": {"array": d_e},
+                      "receptor": 0})
+        d_i = (rng.integers(1, 21, n * k_i) * 0.1).astype(np.float32)
+        ngpu.Connect(inh, net, {"rule": "fixed_indegree", "indegree": k_i},
+                     {"weight": x["w_i"], "delay": {"array": d_i},
+                      "receptor": 1})
+        ngpu.SetStatus(pg[j:j + 1], {"rate": x["rate"]})
+        ngpu.Connect(pg[j:j + 1], net, {"rule": "all_to_all"},
+                     {"weight": w_p, "delay": 0.1, "receptor": 0})
+        ranges.append((i0, i0 + n))
+        n_syn += n * (k_e + k_i)
+        i0 += n
+    return neuron, ranges, n_syn

@@ -333,3 +333,76 @@ def oracle_times_by_neuron(net, dt, lo, hi):
             t = float(np.float64(np.float32(dt)) * (s + 1))
             out[n - lo] += [t] * int(c)
     return [np.array(x) for x in out]
+
+
+def ensemble_instance_spec(n, seed, n_gen=40, k_rec=60, rate_hz=150.0,
+                           t_max=300.0, w_gen=16.0):
+    """One small AdEx network for the ensemble tests: n aeif_cond_beta
+    neurons (2 ports), k_rec recurrent inputs each (80 % E with weight
+    0.0625 nS on port 0, 20 % I with 0.375 nS on port 1, delays 1-20
+    steps), driven by n_gen spike generators with seeded Poisson-like
+    spike times (rate_hz) that connect to 30 random neurons each with
+    weight w_gen nS.  All weights are dyadic, so every synaptic sum is exact
+    in any order."""
+    rng = np.random.default_rng(seed)
+    n_e = int(0.8 * n)
+    src, tgt, w, d, port = [], [], [], [], []
+    for j in range(n):
+        se = rng.choice(n_e, int(0.8 * k_rec), replace=False)
+        si = n_e + rng.choice(n - n_e, k_rec - int(0.8 * k_rec),
+                              replace=False)
+        src += [se, si]
+        tgt += [np.full(se.size, j), np.full(si.size, j)]
+        w += [np.full(se.size, 0.0625), np.full(si.size, 0.375)]
+        d += [rng.integers(1, 21, se.size), rng.integers(1, 21, si.size)]
+        port += [np.zeros(se.size, int), np.ones(si.size, int)]
+    src = np.concatenate(src).astype(np.int32)
+    tgt = np.concatenate(tgt).astype(np.int32)
+    w = np.concatenate(w).astype(np.float32)
+    d = (np.concatenate(d) * 0.1).astype(np.float32)
+    port = np.concatenate(port)
+    gens = []
+    g_src, g_tgt = [], []
+    for g in range(n_gen):
+        k = rng.poisson(rate_hz * t_max * 1e-3)
+        times = np.unique(np.round(rng.uniform(1.0, t_max, k), 1))
+        gens.append(times)
+        tg = rng.choice(n, 30, replace=False)
+        g_src.append(np.full(30, n + g))
+        g_tgt.append(tg)
+    return {"n": n, "n_gen": n_gen, "w_gen": w_gen,
+            "rec": (src, tgt, w, d, port),
+            "gen": (np.concatenate(g_src).astype(np.int32),
+                    np.concatenate(g_tgt).astype(np.int32)),
+            "gen_times": gens}
+
+
+def ensemble_spec(instances):
+    """Spec of several instances in one aeif group and one spike-generator
+    group (node layout: all neurons, then all generators)."""
+    n_tot = sum(x["n"] for x in instances)
+    conns, spike_gen = [], []
+    off, goff = 0, n_tot
+    for x in instances:
+        src, tgt, w, d, port = x["rec"]
+        for p in (0, 1):
+            m = port == p
+            conns.append({"src": src[m] + off, "tgt": tgt[m] + off,
+                          "weight": w[m], "delay": d[m], "port": p})
+        gs, gt = x["gen"]
+        conns.append({"src": gs - x["n"] + goff, "tgt": gt + off,
+                      "weight": np.float32(x["w_gen"]),
+                      "delay": np.float32(0.1),
+                      "port": 0})
+        for g, times in enumerate(x["gen_times"]):
+            spike_gen.append({"node": goff + g, "times": times})
+        off += x["n"]
+        goff += x["n_gen"]
+    params = {"E_rev": [0.0, -85.0], "tau_rise": [0.2, 0.5],
+              "tau_decay": [2.0, 10.0]}
+    return {"dt": 0.1, "seed": 99,
+            "groups": [{"model": "aeif_cond_beta", "n": n_tot, "n_port": 2,
+                        "params": params, "rk5": {"rk5_tol": 1e-3}},
+                       {"model": "spike_generator",
+                        "n": sum(x["n_gen"] for x in instances)}],
+            "conns": conns, "spike_gen": spike_gen}

@@ -310,3 +310,57 @@ def test_c3_stdp_network_statistics(ngpu):
     assert wg.min() >= 0.0 and wg.max() <= 0.1 + 1e-7
     assert abs(wg.mean() - wo.mean()) <= 0.005 * wo.mean()
     assert abs(wg.std() - wo.std()) <= 0.005 * wo.std()
+
+
+def _ensemble_instances():
+    return [nb.ensemble_instance_spec(n, seed)
+            for n, seed in ((300, 11), (700, 12), (450, 13))]
+
+
+def _run_spikes(ngpu, spec, t_ms, settings=None):
+    groups, _ = nb.build_engine(ngpu, spec)
+    for k, v in (settings or {}).items():
+        ngpu.SetKernelStatus(k, v)
+    ngpu.ActivateRecSpikeTimes(groups[0], 2000)
+    ngpu.Simulate(t_ms)
+    return nb.engine_spikes(ngpu, groups[0])
+
+
+def test_ensemble_routing_matches_instances_alone(ngpu):
+    """Three AdEx networks of different sizes batched into one node group
+    (one launch per kernel per step) spike exactly as each network run
+    alone: same synapses, same delay slots, same inputs, node indices
+    shifted by the instance offset (dyadic weights make every sum exact)."""
+    log("case ensemble_routing")
+    inst = _ensemble_instances()
+    ens = sorted(_run_spikes(ngpu, nb.ensemble_spec(inst), 300.0))
+    off = 0
+    total = 0
+    for x in inst:
+        alone = sorted((i + off, t) for i, t in
+                       _run_spikes(ngpu, nb.ensemble_spec([x]), 300.0))
+        part = sorted(s for s in ens if off <= s[0] < off + x["n"])
+        log("instance", x["n"], "spikes alone", len(alone), "batched",
+            len(part))
+        assert len(alone) > 50
+        assert part == alone
+        total += len(alone)
+        off += x["n"]
+    assert len(ens) == total
+
+
+@pytest.mark.parametrize("settings", [
+    {"rk5_two_pass": 0},
+    {"rk5_budget": 3, "rk5_pass2_grid": 64},
+    {"delivery_grid": 256, "delivery_tile_max": 8, "fuse_delivery": 0},
+    {"delivery_grid": 4096, "delivery_tile_max": 1},
+])
+def test_launch_shapes_do_not_change_results(ngpu, settings):
+    """The autotuner's choices (RK5 two-pass budget and grid, delivery grid
+    and tile size, fusion) leave every spike unchanged."""
+    log("case launch_shapes", settings)
+    spec = nb.ensemble_spec(_ensemble_instances())
+    ref = sorted(_run_spikes(ngpu, spec, 200.0))
+    got = sorted(_run_spikes(ngpu, spec, 200.0, settings))
+    assert len(ref) > 100
+    assert got == ref

@@ -1,0 +1,94 @@
+"""Learned RK5 step-size proposer: FP32 training kernels against a numpy
+FP64 restatement of the same MLP, convergence on a known function, and
+the end-to-end invariant (the RK5 error test still judges every attempt,
+so network statistics stay those of the plain controller)."""
+import sys
+
+import numpy as np
+import pytest
+
+import netbuild as nb
+
+pytestmark = pytest.mark.gpu
+
+NI, NH = 6, 32
+W1, B1 = 0, NH * NI
+W2 = B1 + NH
+B2 = W2 + NH * NH
+W3 = B2 + NH
+B3 = W3 + NH
+NP = B3 + 1
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mlp_loss_grad(w, x, t):
+    """Mean of 0.5 (y - t)^2 over the batch and its gradient (FP64)."""
+    w = w.astype(np.float64)
+    A1 = w[W1:B1].reshape(NH, NI)
+    b1 = w[B1:W2]
+    A2 = w[W2:B2].reshape(NH, NH)
+    b2 = w[B2:W3]
+    a3 = w[W3:B3]
+    b3 = w[B3]
+    z1 = x @ A1.T + b1
+    h1 = np.maximum(z1, 0)
+    z2 = h1 @ A2.T + b2
+    h2 = np.maximum(z2, 0)
+    y = h2 @ a3 + b3
+    dy = (y - t) / len(t)
+    g = np.zeros(NP)
+    g[B3] = dy.sum()
+    g[W3:B3] = h2.T @ dy
+    d2 = np.outer(dy, a3) * (z2 > 0)
+    g[B2:W3] = d2.sum(0)
+    g[W2:B2] = (d2.T @ h1).ravel()
+    d1 = (d2 @ A2) * (z1 > 0)
+    g[B1:W2] = d1.sum(0)
+    g[W1:B1] = (d1.T @ x).ravel()
+    return 0.5 * np.mean((y - t) ** 2), g
+
+
+def test_training_step_matches_numpy(ngpu):
+    """One full-batch SGD step of the HIP training kernels equals
+    w - lr * grad of the FP64 restatement."""
+    log("case proposer_grad")
+    rng = np.random.default_rng(5)
+    w0 = rng.normal(0.0, 0.3, NP).astype(np.float32)
+    x = rng.normal(0.0, 1.0, (512, NI)).astype(np.float32)
+    t = rng.normal(0.0, 1.0, 512).astype(np.float32)
+    smp = np.zeros((512, 8), np.float32)
+    smp[:, :NI] = x
+    smp[:, NI] = t
+    ngpu.ProposerSetWeights(w0)
+    ngpu.ProposerSetSamples(smp)
+    lr = 0.05
+    loss = ngpu.ProposerTrain(1, batch=512, lr=lr, adam=False,
+                              full_batch=True)
+    w1 = ngpu.ProposerGetWeights()
+    l_ref, g = mlp_loss_grad(w0, x.astype(np.float64), t.astype(np.float64))
+    want = w0.astype(np.float64) - lr * g
+    log("loss", loss, "numpy", l_ref, "max |dw|",
+        np.max(np.abs(w1 - want)))
+    assert abs(loss - l_ref) <= 1e-4 * l_ref
+    assert np.allclose(w1, want, rtol=1e-4, atol=2e-6)
+
+
+def test_training_fits_known_function(ngpu):
+    """Adam on samples of t = 0.5 x0 - 0.3 x4 + 0.2 |x1| reaches a mean
+    squared error below 2e-3."""
+    log("case proposer_fit")
+    rng = np.random.default_rng(7)
+    x = rng.normal(0.0, 1.0, (100000, NI)).astype(np.float32)
+    t = 0.5 * x[:, 0] - 0.3 * x[:, 4] + 0.2 * np.abs(x[:, 1])
+    smp = np.zeros((x.shape[0], 8), np.float32)
+    smp[:, :NI] = x
+    smp[:, NI] = t
+    ngpu.ProposerSetSamples(smp)
+    loss0 = ngpu.ProposerTrain(1, batch=4096, lr=1e-3)
+    loss = ngpu.ProposerTrain(3000, batch=4096, lr=3e-3)
+    log("loss", loss0, "->", loss)
+    assert loss < 2e-3

@@ -288,6 +288,7 @@ def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps,
         / 1e9}
     res["n_neurons"] = neuron.n
     res["n_synapses"] = n_syn
+    res["steps"] = steps
     if extra is not None:
         res["after_warmup"] = extra
     return res
@@ -304,6 +305,81 @@ def profiled_traffic(name="aeif_rk5_c1.md"):
         return None
     m = re.search(r"Scaled traffic: ([0-9.]+) MB per launch", txt)
     return float(m.group(1)) * 1e6 if m else None
+
+
+def learned_vs_plain(ngpu, build, steps, warmup, max_samples=10000000,
+                     max_log_steps=4000, train_iter=5000, batch=8192):
+    """Train the learned RK5 step-size proposer on samples logged from the
+    plain controller of the running network (after its plain measurement),
+    then rebuild the same network, warm it up with the plain controller and
+    time `steps` steps with the proposer on: the same simulated window as
+    the plain measurement."""
+    dt = ngpu.GetTimeResolution()
+    t = time.time()
+    ngpu.ProposerLog(1, max_samples)
+    done = 0
+    while done < max_log_steps and ngpu.ProposerNSamples() < max_samples:
+        ngpu.Simulate(200 * dt)
+        done += 200
+    ngpu.ProposerLog(0, 0)
+    n_smp = ngpu.ProposerNSamples()
+    t_log = time.time() - t
+    t = time.time()
+    loss = ngpu.ProposerTrain(train_iter, batch=batch, lr=1e-3)
+    t_train = time.time() - t
+    w = ngpu.ProposerGetWeights()
+    log("  proposer: %d samples in %d steps (%.1f s), %d Adam steps of %d "
+        "in %.1f s, loss %.4f" % (n_smp, done, t_log, train_iter, batch,
+                                  t_train, loss))
+    neuron, _ = build()
+    ngpu.ActivateSpikeCount(neuron)
+    ngpu.Calibrate()
+    if warmup > 0:
+        ngpu.Simulate(warmup * dt)
+    ngpu.ProposerSetWeights(w)
+    ngpu.ProposerUse(1)
+    s0 = ngpu.GetHotPathStats()
+    c0 = int(np.sum(ngpu.GetNeuronIntVar(neuron.i0, neuron.n, "spike_count")))
+    hip_sync()
+    t0 = time.perf_counter()
+    ngpu.Simulate(steps * dt)
+    hip_sync()
+    wall = time.perf_counter() - t0
+    s1 = ngpu.GetHotPathStats()
+    c1 = int(np.sum(ngpu.GetNeuronIntVar(neuron.i0, neuron.n, "spike_count")))
+    ngpu.ProposerUse(0)
+    acc = s1["rk5_accepted"] - s0["rk5_accepted"]
+    rej = s1["rk5_rejected"] - s0["rk5_rejected"]
+    bio_s = steps * dt * 1e-3
+    return {"samples": n_smp, "log_steps": done, "train_iter": train_iter,
+            "batch": batch, "train_s": round(t_train, 2),
+            "train_loss": round(loss, 5),
+            "stages_per_neuron_step": (6.0 * acc + 5.0 * rej) /
+            (neuron.n * steps),
+            "rejected_per_neuron_step": rej / (neuron.n * steps),
+            "wall_s_per_bio_s": wall / bio_s,
+            "rate_hz": (c1 - c0) / neuron.n / bio_s}
+
+
+def stage_comparison(plain, learned):
+    p = plain["rk5_stages_per_neuron_step"]
+    q = learned["stages_per_neuron_step"]
+    return {"plain": round(p, 5), "learned": round(q, 5),
+            "reduction_pct": round(100.0 * (1.0 - q / p), 2),
+            "reduction_beyond_one_attempt_pct": round(
+                100.0 * (1.0 - (q - 6.0) / (p - 6.0)), 2) if p > 6.0 else None,
+            "rejected_per_neuron_step": {
+                "plain": round(plain["rk5_rejected"] /
+                               (plain["n_neurons"] * plain["steps"]), 6),
+                "learned": round(learned["rejected_per_neuron_step"], 6)},
+            "rate_hz": {"plain": round(plain["rate_hz"], 3),
+                        "learned": round(learned["rate_hz"], 3)},
+            "wall_s_per_bio_s": {
+                "plain": round(plain["wall_s_per_bio_s"], 4),
+                "learned": round(learned["wall_s_per_bio_s"], 4)},
+            "proposer": {k: learned[k] for k in
+                         ("samples", "log_steps", "train_iter", "batch",
+                          "train_s", "train_loss")}}
 
 
 def roofline(res, traffic=None):
@@ -458,7 +534,11 @@ def main():
         rd = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
                      args.timing_steps)
         log("readme:", json.dumps(rd))
+        lr_ = learned_vs_plain(ngpu, lambda: build_readme(ngpu, args.neurons),
+                               args.steps, args.warmup)
+        log("readme learned:", json.dumps(lr_))
         out["readme_network"] = {
+            "rk5_stages_learned_vs_plain": stage_comparison(rd, lr_),
             "workload": "reference README benchmark network "
                         "(c++/examples/brunel_net.cpp, 1M neurons, 1e9 "
                         "synapses, 20 kHz Poisson, RK5 tol 1e-6)",
@@ -500,7 +580,15 @@ def main():
         r3 = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
                      args.timing_steps)
         log("c3:", json.dumps(r3))
+        l3 = learned_vs_plain(ngpu, lambda: build_c3(ngpu, args.c3_scale),
+                              args.steps, args.warmup)
+        log("c3 learned:", json.dumps(l3))
+        cmp3 = stage_comparison(r3, l3)
+        out["rk5_stages_learned_vs_plain"] = dict(
+            workload="c3", plain=cmp3["plain"], learned=cmp3["learned"],
+            reduction_pct=cmp3["reduction_pct"])
         out["c3"] = {
+            "rk5_stages_learned_vs_plain": cmp3,
             "workload": "c3: 2e5 aeif_cond_alpha (80 % E), fixed in-degree "
                         "500, STDP on E->E (tau 20 ms, lambda 1e-3, alpha "
                         "1.05, Wmax 0.1 nS, w0 ~ U(0.02, 0.06)), own 8 kHz "

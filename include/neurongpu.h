@@ -486,6 +486,7 @@ class NeuronGPU {
   int ProposerGetWeights(float *w);
   int ProposerSetWeights(const float *w);
   int ProposerUse(int on);
+  int ProposerEval(const float *feat, int n, float *out);
 };
 
 #endif

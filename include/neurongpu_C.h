@@ -219,6 +219,9 @@ extern "C" {
   /* on != 0: the second RK5 pass starts its steps from the proposer's step
    * size instead of the plain controller's */
   int NeuronGPU_ProposerUse(int on);
+  /* the proposer's output (log2 of the step in units of dt, before the
+   * safety factor) for n feature vectors feat[n][6] */
+  int NeuronGPU_ProposerEval(float *feat, int n, float *out);
 
 #ifdef __cplusplus
 }

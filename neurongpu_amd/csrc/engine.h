@@ -193,6 +193,7 @@ class Engine {
   // learned RK5 step-size proposer
   DBuf<float> d_prop_w, d_prop_m, d_prop_v, d_prop_grad, d_prop_loss;
   DBuf<uint16_t> d_prop_bf16;
+  DBuf<unsigned char> d_prop_pk;   // MFMA-layout inference copy
   DBuf<float> d_prop_log;
   DBuf<unsigned long long> d_prop_log_n;
   unsigned long long prop_log_max = 0;

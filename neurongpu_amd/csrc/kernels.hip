@@ -463,32 +463,87 @@ __device__ __forceinline__ void prop_features(const AeifLocal<P> &p,
   f[5] = __log2f(e) * (0.30103f * 0.25f);
 }
 
-__device__ __forceinline__ float bf16f(uint16_t u) {
-  return __uint_as_float((uint32_t)u << 16);
+typedef short prop_s4 __attribute__((ext_vector_type(4)));
+typedef float prop_f16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ short prop_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (short)(u >> 16);
 }
 
-__device__ __forceinline__ float prop_mlp(const uint16_t *__restrict__ w,
-                                          const float (&x)[kPropIn]) {
-  float h1[kPropHid];
+// The MLP for the 64 samples of a wavefront (one per lane) on the matrix
+// cores: H1^T = W1 X^T and H2^T = W2 H1^T as 32x32x8 BF16 MFMAs with FP32
+// accumulation, two 32-sample column blocks each; the accumulator layout of
+// a layer is the B-operand layout of the next, so only the input features
+// cross lanes.  Every lane of the wave must call it (wave-uniform branch);
+// returns the output for the calling lane's own sample.
+__device__ __forceinline__ float prop_mlp_mfma(const unsigned char *pk,
+                                               const float (&f)[kPropIn]) {
+  const uint32_t l = lane_id();
+  const bool hi = l >= 32;
+  float x[8], xs[8];
 #pragma unroll
-  for (int j = 0; j < kPropHid; j++) {
-    float a = bf16f(w[kPropB1 + j]);
+  for (int j = 0; j < 8; j++) x[j] = j < kPropIn ? f[j] : 0.0f;
 #pragma unroll
-    for (int i = 0; i < kPropIn; i++)
-      a = __builtin_fmaf(bf16f(w[kPropW1 + j * kPropIn + i]), x[i], a);
-    h1[j] = a > 0.0f ? a : 0.0f;
+  for (int j = 0; j < 8; j++) xs[j] = __shfl_xor(x[j], 32);
+  prop_s4 b0, b1;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    b0[j] = prop_bf16(hi ? xs[4 + j] : x[j]);  // samples 0-31
+    b1[j] = prop_bf16(hi ? x[4 + j] : xs[j]);  // samples 32-63
   }
-  float out = bf16f(w[kPropB3]);
+  const prop_s4 a1 = reinterpret_cast<const prop_s4 *>(pk + kPropPkW1)[l];
+  const float *bp1 = reinterpret_cast<const float *>(pk + kPropPkB1) + l * 16;
+  const float *bp2 = reinterpret_cast<const float *>(pk + kPropPkB2) + l * 16;
+  const float *wp3 = reinterpret_cast<const float *>(pk + kPropPkW3) + l * 16;
+  prop_f16 c1, c2;
 #pragma unroll
-  for (int j = 0; j < kPropHid; j++) {
-    float a = bf16f(w[kPropB2 + j]);
-#pragma unroll
-    for (int i = 0; i < kPropHid; i++)
-      a = __builtin_fmaf(bf16f(w[kPropW2 + j * kPropHid + i]), h1[i], a);
-    a = a > 0.0f ? a : 0.0f;
-    out = __builtin_fmaf(bf16f(w[kPropW3 + j]), a, out);
+  for (int r = 0; r < 16; r++) {
+    c1[r] = bp1[r];
+    c2[r] = bp2[r];
   }
-  return out;
+  prop_f16 d0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b0, c1, 0, 0, 0);
+  prop_f16 d1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, b1, c1, 0, 0, 0);
+  prop_f16 e0 = c2, e1 = c2;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const prop_s4 a2 =
+        reinterpret_cast<const prop_s4 *>(pk + kPropPkW2)[c * 64 + l];
+    prop_s4 h0, h1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      float u0 = d0[4 * c + j], u1 = d1[4 * c + j];
+      h0[j] = prop_bf16(u0 > 0.0f ? u0 : 0.0f);
+      h1[j] = prop_bf16(u1 > 0.0f ? u1 : 0.0f);
+    }
+    e0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, h0, e0, 0, 0, 0);
+    e1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, h1, e1, 0, 0, 0);
+  }
+  float p0 = 0.0f, p1 = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    float w = wp3[r];
+    p0 = __builtin_fmaf(w, e0[r] > 0.0f ? e0[r] : 0.0f, p0);
+    p1 = __builtin_fmaf(w, e1[r] > 0.0f ? e1[r] : 0.0f, p1);
+  }
+  p0 += __shfl_xor(p0, 32);
+  p1 += __shfl_xor(p1, 32);
+  return (hi ? p1 : p0) + *reinterpret_cast<const float *>(pk + kPropPkB3);
+}
+
+// Inference on given feature vectors (n padded to whole waves); every
+// lane runs the MLP, lanes beyond n use zero features and discard the
+// output.
+__global__ __launch_bounds__(kBlock) void prop_eval_kernel(
+    const unsigned char *pk, const float *feat, int n, float *out) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  float f[kPropIn];
+#pragma unroll
+  for (int k = 0; k < kPropIn; k++)
+    f[k] = i < n ? feat[(size_t)i * kPropIn + k] : 0.0f;
+  float o = prop_mlp_mfma(pk, f);
+  if (i < n) out[i] = o;
 }
 
 constexpr int kStragLists = 64;  // sub-lists of resume records
@@ -696,6 +751,14 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
                                                 uint32_t vgrid) {
   constexpr int N = 2 + 2 * P;
   __shared__ uint32_t s_pre[kStragLists + 1];
+  // proposer weights, staged once per block (read at every step start)
+  __shared__ __attribute__((aligned(16))) unsigned char s_pk[kPropPkBytes];
+  if (A.prop_pk) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(A.prop_pk);
+    uint4 *dst = reinterpret_cast<uint4 *>(s_pk);
+    for (int k = threadIdx.x; k < kPropPkBytes / 16; k += kBlock)
+      dst[k] = src[k];
+  }
   if (threadIdx.x < 64) {  // one wave: exclusive scan of the 64 lengths
     uint32_t c = A.strag_n[threadIdx.x];
     uint32_t incl = c;
@@ -757,15 +820,28 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
     if (__ballot(have) == 0ull) break;
     bool finished = false, attempted = false;
     float feat[kPropIn];
+#pragma unroll
+    for (int k = 0; k < kPropIn; k++) feat[k] = 0.0f;
     float err = 1.0f, hh = 0.0f;
-    if (have) {
-      if (need_k1) {
-        if (A.prop_w) {
-          prop_features<P>(p, y, h, err_last, 1.0f / A.dt, feat);
-          float o = prop_mlp(A.prop_w, feat);
+    if (A.prop_pk) {
+      // learned step size for the lanes that start an RK5 step; the MLP
+      // runs on the whole wave
+      bool want = have && need_k1;
+      if (want) prop_features<P>(p, y, h, err_last, 1.0f / A.dt, feat);
+      if (__ballot(want) != 0ull) {
+        float o = prop_mlp_mfma(s_pk, feat);
+        if (want) {
+          // trust region around the plain controller's step
           float hp = A.dt * exp2f(o) * kPropSafety;
+          float lo = h * kPropShrinkMax, hi = h * kPropGrowMax;
+          hp = hp > lo ? hp : lo;   // also replaces a NaN output
+          hp = hp < hi ? hp : hi;
           h = hp > A.h_min ? hp : A.h_min;
         }
+      }
+    }
+    if (have) {
+      if (need_k1) {
         aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
         need_k1 = false;
       }
@@ -1223,6 +1299,14 @@ void launch_aeif_pass2_p(const AeifArgs &a, const DeliveryArgs *d,
 }
 
 }  // namespace
+
+void LaunchPropEval(const unsigned char *pk, const float *feat, int n,
+                    float *out, void *stream) {
+  int grid = (n + kBlock - 1) / kBlock;
+  if (grid > 0)
+    hipLaunchKernelGGL(prop_eval_kernel, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, pk, feat, n, out);
+}
 
 void LaunchAeifPass2(const AeifArgs &a, int n_port, const DeliveryArgs *d,
                      int d_grid, void *stream) {

@@ -474,4 +474,6 @@ int NeuronGPU_ProposerSetWeights(float *w1313)
     RET(int, 0, ret = instance->ProposerSetWeights(w1313))
 int NeuronGPU_ProposerUse(int on)
     RET(int, 0, ret = instance->ProposerUse(on))
+int NeuronGPU_ProposerEval(float *feat, int n, float *out)
+    RET(int, 0, ret = instance->ProposerEval(feat, n, out))
 }

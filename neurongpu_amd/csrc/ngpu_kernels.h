@@ -109,10 +109,10 @@ struct AeifArgs {
   int budget;
   int pass2_grid;
   int uniform;                // every parameter PField of the group uniform
-  // learned step-size proposer, second pass only: BF16 weights (nullptr:
-  // plain controller) and the sample log of (features, target) per attempt
-  // (nullptr: no logging)
-  const uint16_t *prop_w;
+  // learned step-size proposer, second pass only: weights in the MFMA
+  // lane layout (nullptr: plain controller) and the sample log of
+  // (features, target) per attempt (nullptr: no logging)
+  const unsigned char *prop_pk;
   float *prop_log;
   unsigned long long *prop_log_n;
   unsigned long long prop_log_max;
@@ -210,6 +210,9 @@ struct RecordArgs {
 
 // launchers (kernels.hip); all asynchronous on `stream`
 void LaunchAeif(const AeifArgs &a, int n_port, void *stream);
+// learned step-size proposer on n feature vectors ([n][6]) -> out[n]
+void LaunchPropEval(const unsigned char *pk, const float *feat, int n,
+                    float *out, void *stream);
 // second pass of the two-pass update; with d != nullptr the delivery of
 // the step (non-plastic) runs in the same launch on d_grid more blocks
 void LaunchAeifPass2(const AeifArgs &a, int n_port, const DeliveryArgs *d,

@@ -25,6 +25,27 @@ constexpr int kPropNParam = kPropB3 + 1;                // 1313
 constexpr int kPropSample = 8;   // floats per logged sample: 6 features,
                                  // target, spare
 constexpr float kPropSafety = 0.9f;
+// trust region of a proposal relative to the plain controller's step
+constexpr float kPropShrinkMax = 0.5f;
+constexpr float kPropGrowMax = 32.0f;
+
+// Inference copy in the lane layout of the 32x32x8 BF16 MFMA
+// (v_mfma_f32_32x32x8_bf16): the hidden layers are computed as
+// H^T = W X^T with the 64 samples of a wavefront as two 32-column blocks.
+//   w1p[64][4]      bf16  A operand of layer 1: lane l holds
+//                         W1[l % 32][4 (l / 32) + j], inputs 6, 7 zero
+//   w2p[4][64][4]   bf16  A operands of layer 2, K chunk c: lane l holds
+//                         W2[l % 32][8 c + 4 (l / 32) + j]
+//   b1p, b2p, w3p [64][16] f32  in the accumulator layout: lane l, value r
+//                         belongs to hidden unit 8 (r / 4) + 4 (l / 32) + r % 4
+//   b3               f32
+constexpr int kPropPkW1 = 0;                 // bytes
+constexpr int kPropPkW2 = kPropPkW1 + 64 * 4 * 2;
+constexpr int kPropPkB1 = kPropPkW2 + 4 * 64 * 4 * 2;
+constexpr int kPropPkB2 = kPropPkB1 + 64 * 16 * 4;
+constexpr int kPropPkW3 = kPropPkB2 + 64 * 16 * 4;
+constexpr int kPropPkB3 = kPropPkW3 + 64 * 16 * 4;
+constexpr int kPropPkBytes = kPropPkB3 + 16;
 
 // Features of a neuron at the start of an RK5 step, from quantities the
 // plain controller has at hand:
@@ -54,5 +75,7 @@ struct PropTrainArgs {
 
 const char *LaunchPropTrainStep(const PropTrainArgs &a, void *stream);
 void LaunchPropToBf16(const float *w, uint16_t *w_bf16, void *stream);
+// FP32 master weights -> MFMA-layout inference copy (kPropPkBytes)
+void LaunchPropPack(const float *w, unsigned char *packed, void *stream);
 
 }  // namespace ngpu

@@ -129,6 +129,32 @@ __global__ void update_kernel(PropTrainArgs A, uint32_t step) {
   A.grad[k] = 0.0f;
 }
 
+__global__ void pack_kernel(const float *w, unsigned char *pk) {
+  int l = threadIdx.x;  // 64 threads
+  if (l >= 64) return;
+  uint16_t *w1p = reinterpret_cast<uint16_t *>(pk + kPropPkW1);
+  uint16_t *w2p = reinterpret_cast<uint16_t *>(pk + kPropPkW2);
+  float *b1p = reinterpret_cast<float *>(pk + kPropPkB1);
+  float *b2p = reinterpret_cast<float *>(pk + kPropPkB2);
+  float *w3p = reinterpret_cast<float *>(pk + kPropPkW3);
+  int row = l % 32, half = l / 32;
+  for (int j = 0; j < 4; j++) {
+    int k = 4 * half + j;
+    w1p[l * 4 + j] = k < kPropIn ? to_bf16(w[kPropW1 + row * kPropIn + k])
+                                 : (uint16_t)0;
+    for (int c = 0; c < 4; c++)
+      w2p[(c * 64 + l) * 4 + j] =
+          to_bf16(w[kPropW2 + row * kPropHid + 8 * c + 4 * half + j]);
+  }
+  for (int r = 0; r < 16; r++) {
+    int unit = 8 * (r / 4) + 4 * half + (r % 4);
+    b1p[l * 16 + r] = w[kPropB1 + unit];
+    b2p[l * 16 + r] = w[kPropB2 + unit];
+    w3p[l * 16 + r] = w[kPropW3 + unit];
+  }
+  if (l == 0) *reinterpret_cast<float *>(pk + kPropPkB3) = w[kPropB3];
+}
+
 __global__ void to_bf16_kernel(const float *w, uint16_t *o) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < kPropNParam) o[k] = to_bf16(w[k]);
@@ -144,6 +170,11 @@ const char *LaunchPropTrainStep(const PropTrainArgs &a, void *stream) {
   hipLaunchKernelGGL(update_kernel, dim3((kPropNParam + 255) / 256), dim3(256),
                      0, s, a, a.iter + 1u);
   return nullptr;
+}
+
+void LaunchPropPack(const float *w, unsigned char *packed, void *stream) {
+  hipLaunchKernelGGL(pack_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                     w, packed);
 }
 
 void LaunchPropToBf16(const float *w, uint16_t *w_bf16, void *stream) {

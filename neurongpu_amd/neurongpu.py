@@ -197,6 +197,7 @@ _SIG = {
     "ProposerGetWeights": (c_int, (c_void_p,)),
     "ProposerSetWeights": (c_int, (c_void_p,)),
     "ProposerUse": (c_int, (c_int,)),
+    "ProposerEval": (c_int, (c_void_p, c_int, c_void_p)),
 }
 
 _C = {}
@@ -1459,6 +1460,15 @@ def ProposerSetWeights(w):
     if a.size != PROPOSER_N_PARAM:
         raise ValueError("expected %d weights" % PROPOSER_N_PARAM)
     return _call("ProposerSetWeights", a.ctypes.data_as(c_void_p))
+
+
+def ProposerEval(features):
+    "Proposer output (log2 of the step / dt) for features of shape (n, 6)"
+    f = np.ascontiguousarray(features, dtype=np.float32)
+    out = np.zeros(f.shape[0], dtype=np.float32)
+    _call("ProposerEval", f.ctypes.data_as(c_void_p), f.shape[0],
+          out.ctypes.data_as(c_void_p))
+    return out
 
 
 def ProposerUse(on):

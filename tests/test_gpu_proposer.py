@@ -92,3 +92,34 @@ def test_training_fits_known_function(ngpu):
     loss = ngpu.ProposerTrain(3000, batch=4096, lr=3e-3)
     log("loss", loss0, "->", loss)
     assert loss < 2e-3
+
+
+def bf16(a):
+    """Round FP32 values to BF16 (nearest even) and back."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def test_mfma_inference_matches_numpy(ngpu):
+    """The matrix-core inference (BF16 weights and activations, FP32
+    accumulation) matches a numpy evaluation with the same BF16 rounding:
+    relative error below 5e-3 everywhere (an activation can round to the
+    neighbouring BF16 value when FP32 and FP64 sums straddle a rounding
+    boundary) and below 1e-4 on average, for a sample count that is not a
+    multiple of the wave size."""
+    log("case proposer_infer")
+    rng = np.random.default_rng(9)
+    w = rng.normal(0.0, 0.3, NP).astype(np.float32)
+    ngpu.ProposerSetWeights(w)
+    x = rng.normal(0.0, 1.0, (1000, NI)).astype(np.float32)
+    got = ngpu.ProposerEval(x)
+    A1 = bf16(w[W1:B1]).reshape(NH, NI).astype(np.float64)
+    A2 = bf16(w[W2:B2]).reshape(NH, NH).astype(np.float64)
+    h1 = np.maximum(bf16(x).astype(np.float64) @ A1.T + w[B1:W2], 0)
+    h2 = np.maximum(bf16(h1).astype(np.float64) @ A2.T + w[B2:W3], 0)
+    want = h2 @ w[W3:B3].astype(np.float64) + w[B3]
+    rel = np.abs(got - want) / (1.0 + np.abs(want))
+    log("max rel err", rel.max(), "mean", rel.mean())
+    assert rel.max() < 5e-3
+    assert rel.mean() < 1e-4

@@ -3,8 +3,9 @@
 Picks the kernel launch settings of the per-step path by timing short
 stretches of the running simulation.  The settings are:
 * the first-pass RK5 attempt budget and the second-pass grid;
-* the delivery grid and the largest number of (spike, delay) pairs per
-  wave tile;
+* the delivery mode (one float atomic per synaptic event, or events
+  binned by accumulator range and summed in LDS first), the delivery grid
+  and the largest number of (spike, delay) pairs per wave tile;
 * whether the delivery shares the second RK5 pass's launch.
 
 None of these changes results: each neuron's arithmetic and each synapse's
@@ -17,6 +18,7 @@ import time
 CANDIDATES = {
     "rk5_budget": [1, 2],
     "rk5_pass2_grid": [256, 1024, 4096],
+    "delivery_mode": [0, 1],
     "delivery_grid": [1024, 2048, 4096],
     "delivery_tile_max": [8, 16, 64],
     "fuse_delivery": [1, 0],

@@ -202,6 +202,15 @@ class Engine {
   void PropInit();
   int delivery_grid = 2048;   // blocks of the delivery kernel
   int delivery_tile_max = 64; // largest pairs per wave tile
+  // 0: float atomics per event; 1: binned (events grouped by accumulator
+  // range, summed in LDS); non-plastic networks only
+  int delivery_mode = 0;
+  int delivery_bin_capacity = 1 << 27;  // events in the binned-mode buffer
+  DBuf<uint32_t> d_bin_hist, d_bin_tot, d_bin_bstart, d_bin_cstart;
+  DBuf<unsigned long long> d_bin_ev;
+  DBuf<float> d_bin_part;
+  // sets up the binned-delivery buffers; false if the network cannot use it
+  bool PrepareBins(BinArgs &b);
   int max_port = 1;
   int R = 0;
   long long ring_cap = 0;

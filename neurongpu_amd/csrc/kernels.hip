@@ -1026,12 +1026,15 @@ __global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
 // ---------------------------------------------------------------------
 constexpr int kMaxDelaySlots = 4096;
 
-template <bool PLASTIC>
-__device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
-                                              uint32_t vblock,
-                                              uint32_t vgrid) {
-  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
-  __shared__ uint32_t s_part[kBlock];
+// Block prologue of every delivery kernel: s_pre[d] = number of (spike,
+// delay) pairs with delay slot < d due this step (d = 0..D), from the ring
+// counts of slots gs-1-d.  Virtual block 0 also clears the ring counter of
+// the slot the next step fills (slot gs+1 is never read in this step).
+// Returns the number of pairs.
+__device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
+                                                      uint32_t vblock,
+                                                      uint32_t *s_pre,
+                                                      uint32_t *s_part) {
   const int D = A.D;
   const int tid = threadIdx.x;
   // per-thread partial sums of the due spike counts over d
@@ -1069,14 +1072,26 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
   }
   __syncthreads();
   if (vblock == 0 && tid == 0) A.ring_n[(A.gs + 1) % A.R] = 0u;
-  const uint32_t total = s_pre[D];
+  return s_pre[D];
+}
+
+// Enumerates the synaptic events of this wave's share of the step and calls
+// f(ok, syn, rec, height, mult) for two 64-event chunks at a time (both
+// record loads are issued before the calls).  The wave's tiles of pt pairs
+// (the largest power of two in [1, tile_max] that still gives every wave
+// of the virtual grid a tile) are spread over the 64 lanes by a lane-level
+// scan of the pairs' segment lengths and a binary-lift owner search.  The
+// enumeration depends only on (vblock, vgrid, tile_max) and the ring, so
+// two kernels launched with the same arguments see the same events in the
+// same blocks.  Returns the wave's event count.
+template <class F>
+__device__ __forceinline__ unsigned long long deliver_events(
+    const DeliveryArgs &A, uint32_t vblock, uint32_t vgrid,
+    const uint32_t *s_pre, uint32_t total, F &&f) {
+  const int D = A.D;
   const uint32_t lane = lane_id();
-  const uint32_t wave = (uint32_t)(vblock * (kBlock / 64) + tid / 64);
+  const uint32_t wave = (uint32_t)(vblock * (kBlock / 64) + threadIdx.x / 64);
   const uint32_t n_waves = vgrid * (kBlock / 64);
-  // pairs per wave tile: the largest power of two in [1, tile_max] that
-  // still gives every wave of the grid a tile, so that steps with few
-  // spikes spread their synapses over many waves instead of a few long
-  // loops
   uint32_t pt = (uint32_t)A.tile_max;
   while (pt > 1 && (total + pt - 1) / pt < n_waves) pt >>= 1;
   const uint32_t n_tiles = (total + pt - 1) / pt;
@@ -1114,8 +1129,6 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
     uint32_t pre = incl - cnt;
     uint32_t E = __shfl(incl, 63);
     n_events += E;
-    // two 64-event chunks per iteration: both record loads are issued
-    // before either atomic
     for (uint32_t e0 = 0; e0 < E; e0 += 128u) {
       unsigned long long syn[2], r[2];
       float hj[2];
@@ -1141,34 +1154,48 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
         r[u] = ok[u] ? A.rec[syn[u]] : 0ull;
       }
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
-        if (!ok[u]) continue;
-        uint32_t tp = (uint32_t)r[u];
-        float w = __uint_as_float((uint32_t)(r[u] >> 32));
+      for (int u = 0; u < 2; u++) f(ok[u], syn[u], r[u], hj[u], mj[u]);
+    }
+  }
+  return n_events;
+}
+
+template <bool PLASTIC>
+__device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
+                                              uint32_t vblock,
+                                              uint32_t vgrid) {
+  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
+  __shared__ uint32_t s_part[kBlock];
+  const uint32_t total = delivery_prologue(A, vblock, s_pre, s_part);
+  unsigned long long n_events = deliver_events(
+      A, vblock, vgrid, s_pre, total,
+      [&](bool ok, unsigned long long syn, unsigned long long r, float hj,
+          uint32_t mj) {
+        if (!ok) return;
+        uint32_t tp = (uint32_t)r;
+        float w = __uint_as_float((uint32_t)(r >> 32));
         uint32_t tgt = tp & kTargetMask;
         uint32_t port = tp >> kPortShift;
-        unsafeAtomicAdd(A.acc + (long long)port * A.n_total + tgt, hj[u] * w);
+        unsafeAtomicAdd(A.acc + (long long)port * A.n_total + tgt, hj * w);
         if (PLASTIC) {
-          int sgp = A.syn_group[syn[u]];
+          int sgp = A.syn_group[syn];
           if (sgp > 0) {
             int type = A.syn_type[sgp - 1];
             const float *prm = A.stdp_param + 8 * (sgp - 1);
             float wn = w;
-            for (uint32_t rep = 0; rep < mj[u]; rep++) {
-              A.spike_time[syn[u]] = (uint16_t)(A.T & 0xffff);
+            for (uint32_t rep = 0; rep < mj; rep++) {
+              A.spike_time[syn] = (uint16_t)(A.T & 0xffff);
               long long Dt_int = A.T - A.last_rev[tgt];
               if (Dt_int > 0 && Dt_int < kMaxSynDt)
                 wn = dev_syn_update(type, wn, -A.dt * (float)Dt_int, prm);
             }
             if (wn != w)
-              reinterpret_cast<uint32_t *>(A.rec + syn[u])[1] =
+              reinterpret_cast<uint32_t *>(A.rec + syn)[1] =
                   __float_as_uint(wn);
           }
         }
-      }
-    }
-  }
-  if (lane == 0 && n_events) {
+      });
+  if (lane_id() == 0 && n_events) {
     atomicAdd(A.events + (vblock & (kCounterSlots - 1)), n_events);
   }
 }
@@ -1176,6 +1203,207 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
 template <bool PLASTIC>
 __global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
   delivery_body<PLASTIC>(A, blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------------
+// Binned delivery (non-plastic networks).  Float atomics on the
+// accumulators run at a fixed rate per device whatever the address
+// pattern, so when a step carries many events per accumulator entry the
+// events are first grouped by accumulator range (bin of kBinLen entries):
+//   1 count    per (bin, block) event counts, LDS histogram
+//   2 scan     per bin, exclusive offsets over the blocks, bin totals
+//   3 scatter  the same enumeration again; each event is written as
+//              (index within bin, value) at its bin's cursor; block 0
+//              also writes the bin and chunk starts.  If the step has more
+//              events than the buffer holds, this phase adds them with
+//              atomics instead and the later phases do nothing.
+//   4 reduce   one block per chunk of <= kBinChunk events of one bin sums
+//              them in LDS; a bin of one chunk is added to the
+//              accumulators directly, otherwise the partial sums are
+//              written out
+//   5 sum      bins of several chunks: accumulators += sum of partials.
+// Count and scatter use the same grid and tile settings, so every block
+// enumerates the same events in both.
+// ---------------------------------------------------------------------
+
+// Exclusive scan of val(0..n-1) by one block: out[b] = sum of val(b') for
+// b' < b.  Returns the 64-bit total.  s_part holds kBlock entries.
+template <class F>
+__device__ __forceinline__ unsigned long long block_scan_excl(
+    int n, F val, uint32_t *out, unsigned long long *s_part) {
+  const int tid = threadIdx.x;
+  const int per = (n + kBlock - 1) / kBlock;
+  unsigned long long run = 0;
+  for (int k = 0; k < per; k++) {
+    int b = tid * per + k;
+    if (b < n) run += val(b);
+  }
+  s_part[tid] = run;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    unsigned long long v = (tid >= off) ? s_part[tid - off] : 0ull;
+    __syncthreads();
+    s_part[tid] += v;
+    __syncthreads();
+  }
+  unsigned long long acc = (tid > 0) ? s_part[tid - 1] : 0ull;
+  for (int k = 0; k < per; k++) {
+    int b = tid * per + k;
+    if (b < n) {
+      unsigned long long v = val(b);  // before out[b]: out may alias the input
+      out[b] = (uint32_t)acc;
+      acc += v;
+    }
+  }
+  unsigned long long total = s_part[kBlock - 1];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kBlock) void bin_count_kernel(DeliveryArgs A,
+                                                           BinArgs B) {
+  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
+  __shared__ uint32_t s_part[kBlock];
+  __shared__ uint32_t s_h[kMaxBins];
+  for (int b = threadIdx.x; b < B.nb; b += kBlock) s_h[b] = 0u;
+  const uint32_t total = delivery_prologue(A, blockIdx.x, s_pre, s_part);
+  deliver_events(A, blockIdx.x, gridDim.x, s_pre, total,
+                 [&](bool ok, unsigned long long, unsigned long long r, float,
+                     uint32_t) {
+                   if (!ok) return;
+                   uint32_t tp = (uint32_t)r;
+                   long long idx = (long long)(tp >> kPortShift) * A.n_total +
+                                   (tp & kTargetMask);
+                   atomicAdd(&s_h[idx >> kBinShift], 1u);
+                 });
+  __syncthreads();
+  for (int b = threadIdx.x; b < B.nb; b += kBlock)
+    B.hist[(size_t)b * B.grid + blockIdx.x] = s_h[b];
+}
+
+// one block per bin
+__global__ __launch_bounds__(kBlock) void bin_scan_kernel(BinArgs B) {
+  __shared__ unsigned long long s_part[kBlock];
+  uint32_t *h = B.hist + (size_t)blockIdx.x * B.grid;
+  // in place: each thread reads and rewrites only its own entries
+  unsigned long long t = block_scan_excl(
+      B.grid, [&](int k) -> unsigned long long { return h[k]; }, h, s_part);
+  if (threadIdx.x == 0)
+    B.tot[blockIdx.x] = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+}
+
+__global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A,
+                                                             BinArgs B) {
+  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
+  __shared__ uint32_t s_part[kBlock];
+  __shared__ unsigned long long s_part64[kBlock];
+  __shared__ uint32_t s_cur[kMaxBins];
+  const unsigned long long n_ev = block_scan_excl(
+      B.nb, [&](int b) -> unsigned long long { return B.tot[b]; }, s_cur,
+      s_part64);
+  const bool binned = n_ev <= (unsigned long long)B.ev_cap;
+  if (blockIdx.x == 0) {
+    // bin and chunk starts for the reduce and sum phases
+    for (int b = threadIdx.x; b < B.nb; b += kBlock)
+      B.bstart[b] = binned ? s_cur[b] : 0u;
+    if (threadIdx.x == 0) B.bstart[B.nb] = binned ? (uint32_t)n_ev : 0u;
+    unsigned long long nc = block_scan_excl(
+        B.nb,
+        [&](int b) -> unsigned long long {
+          return binned ? (B.tot[b] + kBinChunk - 1u) / kBinChunk : 0u;
+        },
+        B.cstart, s_part64);
+    if (threadIdx.x == 0) B.cstart[B.nb] = (uint32_t)nc;
+  }
+  for (int b = threadIdx.x; b < B.nb; b += kBlock)
+    s_cur[b] += B.hist[(size_t)b * B.grid + blockIdx.x];
+  // (the prologue's barriers order these LDS writes before their use)
+  const uint32_t total = delivery_prologue(A, blockIdx.x, s_pre, s_part);
+  unsigned long long n_events = deliver_events(
+      A, blockIdx.x, gridDim.x, s_pre, total,
+      [&](bool ok, unsigned long long, unsigned long long r, float hj,
+          uint32_t) {
+        if (!ok) return;
+        uint32_t tp = (uint32_t)r;
+        float w = __uint_as_float((uint32_t)(r >> 32));
+        long long idx = (long long)(tp >> kPortShift) * A.n_total +
+                        (tp & kTargetMask);
+        float v = hj * w;
+        if (binned) {
+          uint32_t pos = atomicAdd(&s_cur[idx >> kBinShift], 1u);
+          B.ev[pos] = ((unsigned long long)__float_as_uint(v) << 32) |
+                      (uint32_t)(idx & (kBinLen - 1));
+        } else {
+          unsafeAtomicAdd(A.acc + idx, v);
+        }
+      });
+  if (lane_id() == 0 && n_events) {
+    atomicAdd(A.events + (blockIdx.x & (kCounterSlots - 1)), n_events);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bin_reduce_kernel(BinArgs B,
+                                                            float *acc) {
+  __shared__ float s_a[kBinLen];
+  const uint32_t n_chunk = B.cstart[B.nb];
+  for (uint32_t w = blockIdx.x; w < n_chunk; w += gridDim.x) {
+    // bin of chunk w: last b with cstart[b] <= w (it has >= 1 chunk)
+    int lo = 0, hi = B.nb;  // cstart[lo] <= w < cstart[hi]
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (B.cstart[mid] <= w) lo = mid; else hi = mid;
+    }
+    const int b = lo;
+    const uint32_t k = w - B.cstart[b];
+    const uint32_t nc = B.cstart[b + 1] - B.cstart[b];
+    const uint32_t e_end = B.bstart[b + 1];
+    const uint32_t e0 = B.bstart[b] + k * (uint32_t)kBinChunk;
+    const uint32_t e1 = (e_end - e0 < (uint32_t)kBinChunk) ? e_end
+                                                         : e0 + kBinChunk;
+    for (int i = threadIdx.x; i < kBinLen; i += kBlock) s_a[i] = 0.0f;
+    __syncthreads();
+    uint32_t e = e0 + threadIdx.x;
+    for (; e + 3u * kBlock < e1; e += 4u * kBlock) {
+      unsigned long long v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = B.ev[e + (uint32_t)u * kBlock];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        atomicAdd(&s_a[(uint32_t)v[u]], __uint_as_float((uint32_t)(v[u] >> 32)));
+    }
+    for (; e < e1; e += kBlock) {
+      unsigned long long v = B.ev[e];
+      atomicAdd(&s_a[(uint32_t)v], __uint_as_float((uint32_t)(v >> 32)));
+    }
+    __syncthreads();
+    const long long a0 = (long long)b * kBinLen;
+    const long long lim_ll = B.acc_n - a0;
+    const int lim = lim_ll < kBinLen ? (int)lim_ll : kBinLen;
+    if (nc == 1) {
+      for (int i = threadIdx.x; i < lim; i += kBlock) {
+        float x = s_a[i];
+        if (x != 0.0f) acc[a0 + i] += x;
+      }
+    } else {
+      float *pp = B.part + (size_t)w * kBinLen;
+      for (int i = threadIdx.x; i < kBinLen; i += kBlock) pp[i] = s_a[i];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bin_sum_kernel(BinArgs B,
+                                                         float *acc) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+       i < B.acc_n; i += (long long)gridDim.x * kBlock) {
+    const int b = (int)(i >> kBinShift);
+    const uint32_t c0 = B.cstart[b], c1 = B.cstart[b + 1];
+    if (c1 - c0 <= 1u) continue;
+    const int j = (int)(i & (kBinLen - 1));
+    float sum = 0.0f;
+    for (uint32_t c = c0; c < c1; c++) sum += B.part[(size_t)c * kBinLen + j];
+    acc[i] += sum;
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -1365,6 +1593,21 @@ void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream) {
   else
     hipLaunchKernelGGL(delivery_kernel<false>, dim3(grid), dim3(kBlock), 0,
                        (hipStream_t)stream, a);
+}
+
+void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,
+                          int reduce_grid, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(bin_count_kernel, dim3(b.grid), dim3(kBlock), 0, s, a, b);
+  hipLaunchKernelGGL(bin_scan_kernel, dim3(b.nb), dim3(kBlock), 0, s, b);
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(b.grid), dim3(kBlock), 0, s, a,
+                     b);
+  hipLaunchKernelGGL(bin_reduce_kernel, dim3(reduce_grid), dim3(kBlock), 0, s,
+                     b, a.acc);
+  long long g = (b.acc_n + kBlock - 1) / kBlock;
+  if (g > 2048) g = 2048;
+  hipLaunchKernelGGL(bin_sum_kernel, dim3((unsigned)g), dim3(kBlock), 0, s, b,
+                     a.acc);
 }
 
 void LaunchRevStdp(const RevArgs &a, int grid, void *stream) {

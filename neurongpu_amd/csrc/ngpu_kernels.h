@@ -178,6 +178,29 @@ struct DeliveryArgs {
   unsigned long long *events;   // [kCounterSlots]
 };
 
+// Binned delivery (non-plastic networks): the step's synaptic events are
+// counted per (block, accumulator range of kBinLen entries), placed
+// range-major into an event buffer, summed per range in LDS and added to
+// the accumulators once per entry instead of once per event.
+constexpr int kBinShift = 13;
+constexpr int kBinLen = 1 << kBinShift;   // accumulator entries per bin
+constexpr int kMaxBins = 8192;            // bins a block can count in LDS
+constexpr int kBinChunk = 65536;          // events summed by one block
+
+struct BinArgs {
+  uint32_t *hist;           // [nb][grid] counts, then offsets within the bin
+  uint32_t *tot;            // [nb] events per bin
+  uint32_t *bstart;         // [nb+1] first event of each bin
+  uint32_t *cstart;         // [nb+1] first chunk of each bin
+  unsigned long long *ev;   // [ev_cap] (index within bin, value bits)
+  float *part;              // [chunk_cap][kBinLen] partial sums
+  long long ev_cap;         // above this many events the step falls back
+                            // to atomic delivery
+  int nb;                   // bins: ceil(acc_n / kBinLen)
+  int grid;                 // blocks of the count and scatter phases
+  long long acc_n;          // accumulator entries (ports x nodes)
+};
+
 struct RevArgs {
   const uint32_t *ring_src;
   long long cap;
@@ -221,6 +244,10 @@ void LaunchIaf(const IafArgs &a, void *stream);
 void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
+// Binned delivery: count, scan, scatter, reduce, sum (five launches).
+// Requires a non-plastic network and b.nb <= kMaxBins.
+void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,
+                          int reduce_grid, void *stream);
 void LaunchRevStdp(const RevArgs &a, int grid, void *stream);
 void LaunchDenRev(const RevArgs &a, int grid, void *stream);
 void LaunchRecord(const RecordArgs &a, void *stream);

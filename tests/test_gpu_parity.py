@@ -255,16 +255,26 @@ def test_network_statistics_vs_fp64(ngpu):
         assert abs(se["corr"] - so["corr"]) <= 0.02
 
 
-def test_c2_iaf_network_bit_exact(ngpu):
+@pytest.mark.parametrize("settings", [
+    {},
+    {"delivery_mode": 1},
+    {"delivery_mode": 1, "delivery_bin_capacity": 50000},
+])
+def test_c2_iaf_network_bit_exact(ngpu, settings):
     """c2-shaped iaf_psc_exp network at 10 % size (7717 neurons, 5e6
     synapses, 8 Poisson populations), weights rounded to whole pA so that
     every synaptic sum is exact: spike trains equal the FP32 oracle's spike
-    for spike over 100 ms."""
+    for spike over 100 ms.  Run with per-event atomics, with binned
+    delivery (about 1e5 events per bin per step, so bins span several
+    reduction chunks) and with a binned-event buffer too small for the
+    busy steps (those fall back to atomics)."""
     from neurongpu_amd import workloads as wl
-    log("case c2_scaled")
+    log("case c2_scaled", settings)
     spec = wl.c2_spec(scale=0.1, integer_weights=True)
     n = spec["groups"][0]["n"]
     groups, _ = nb.build_engine(ngpu, spec)
+    for k, v in settings.items():
+        ngpu.SetKernelStatus(k, v)
     ngpu.ActivateRecSpikeTimes(groups[0], 200)
     ngpu.Simulate(100.0)
     eng = sorted(nb.engine_spikes(ngpu, groups[0]))
@@ -354,10 +364,12 @@ def test_ensemble_routing_matches_instances_alone(ngpu):
     {"rk5_budget": 3, "rk5_pass2_grid": 64},
     {"delivery_grid": 256, "delivery_tile_max": 8, "fuse_delivery": 0},
     {"delivery_grid": 4096, "delivery_tile_max": 1},
+    {"delivery_mode": 1},
+    {"delivery_mode": 1, "delivery_grid": 64, "delivery_tile_max": 4},
 ])
 def test_launch_shapes_do_not_change_results(ngpu, settings):
-    """The autotuner's choices (RK5 two-pass budget and grid, delivery grid
-    and tile size, fusion) leave every spike unchanged."""
+    """The autotuner's choices (RK5 two-pass budget and grid, delivery grid,
+    tile size and mode, fusion) leave every spike unchanged."""
     log("case launch_shapes", settings)
     spec = nb.ensemble_spec(_ensemble_instances())
     ref = sorted(_run_spikes(ngpu, spec, 200.0))

@@ -18,7 +18,7 @@ import time
 CANDIDATES = {
     "rk5_budget": [1, 2],
     "rk5_pass2_grid": [256, 1024, 4096],
-    "delivery_mode": [0, 1],
+    "delivery_mode": [2, 0, 1],
     "delivery_grid": [1024, 2048, 4096],
     "delivery_tile_max": [8, 16, 64],
     "fuse_delivery": [1, 0],

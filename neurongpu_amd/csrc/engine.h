@@ -204,9 +204,19 @@ class Engine {
   int delivery_tile_max = 64; // largest pairs per wave tile
   // 0: float atomics per event; 1: binned (events grouped by accumulator
   // range, summed in LDS); non-plastic networks only
-  int delivery_mode = 0;
+  // 2: auto (binned once the recent steps average delivery_auto_events
+  // synaptic events or more, read from the event counter with a lag)
+  int delivery_mode = 2;
   int delivery_bin_capacity = 1 << 27;  // events in the binned-mode buffer
-  DBuf<uint32_t> d_bin_hist, d_bin_tot, d_bin_bstart, d_bin_cstart;
+  int delivery_auto_events = 500000;
+  unsigned long long *h_ev_snap = nullptr;  // pinned copy of the counters
+  hipEvent_t ev_snap = nullptr;
+  bool snap_pending = false, auto_binned = false;
+  long long snap_gs = -1, prev_snap_gs = -1;
+  unsigned long long prev_snap_total = 0;
+  bool AutoBinned();
+  DBuf<uint32_t> d_bin_hist, d_bin_tot, d_bin_bstart, d_bin_cstart,
+      d_bin_meta;
   DBuf<unsigned long long> d_bin_ev;
   DBuf<float> d_bin_part;
   // sets up the binned-delivery buffers; false if the network cannot use it

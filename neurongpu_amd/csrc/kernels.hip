@@ -1024,7 +1024,7 @@ __global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
 // the tile (consecutive lanes read consecutive 8-byte records) and add
 // height*weight into the target accumulators.
 // ---------------------------------------------------------------------
-constexpr int kMaxDelaySlots = 4096;
+constexpr int kMaxDelaySlots = 4096;  // checked on the host at calibration
 
 // Block prologue of every delivery kernel: s_pre[d] = number of (spike,
 // delay) pairs with delay slot < d due this step (d = 0..D), from the ring
@@ -1076,15 +1076,19 @@ __device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
 }
 
 // Enumerates the synaptic events of this wave's share of the step and calls
-// f(ok, syn, rec, height, mult) for two 64-event chunks at a time (both
-// record loads are issued before the calls).  The wave's tiles of pt pairs
-// (the largest power of two in [1, tile_max] that still gives every wave
-// of the virtual grid a tile) are spread over the 64 lanes by a lane-level
-// scan of the pairs' segment lengths and a binary-lift owner search.  The
+// f(ok, syn, rec, height, mult) for U 64-event chunks at a time (all U
+// record loads are issued before the calls).  The wave takes tiles of pt
+// pairs (the largest power of two in [1, tile_max] that still gives every
+// wave of the virtual grid a tile).  When the tile's segments average at
+// least kLongSegment synapses they are walked one at a time; otherwise
+// the tile's events are spread over the 64 lanes by a lane-level scan of
+// the segment lengths and a binary-lift owner search.  The
 // enumeration depends only on (vblock, vgrid, tile_max) and the ring, so
 // two kernels launched with the same arguments see the same events in the
 // same blocks.  Returns the wave's event count.
-template <class F>
+constexpr uint32_t kLongSegment = 32;
+
+template <int U, class F>
 __device__ __forceinline__ unsigned long long deliver_events(
     const DeliveryArgs &A, uint32_t vblock, uint32_t vgrid,
     const uint32_t *s_pre, uint32_t total, F &&f) {
@@ -1129,13 +1133,48 @@ __device__ __forceinline__ unsigned long long deliver_events(
     uint32_t pre = incl - cnt;
     uint32_t E = __shfl(incl, 63);
     n_events += E;
-    for (uint32_t e0 = 0; e0 < E; e0 += 128u) {
-      unsigned long long syn[2], r[2];
-      float hj[2];
-      uint32_t mj[2];
-      bool ok[2];
+    const unsigned long long nz = __ballot(cnt > 0u);
+    if (E >= (uint32_t)kLongSegment * (uint32_t)__popcll(nz)) {
+      // Long segments: one pair at a time, its data broadcast through
+      // scalar registers, consecutive lanes on consecutive records.
+      unsigned long long m = nz;
+      while (m) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1ull;
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cnt, j);
+        const uint32_t slo =
+            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)start, j);
+        const uint32_t shi =
+            (uint32_t)__builtin_amdgcn_readlane((int)(start >> 32), j);
+        const unsigned long long st = ((unsigned long long)shi << 32) | slo;
+        const float h = __uint_as_float((uint32_t)__builtin_amdgcn_readlane(
+            (int)__float_as_uint(height), j));
+        const uint32_t mu = (uint32_t)__builtin_amdgcn_readlane((int)mult, j);
+        for (uint32_t o = 0; o < c; o += 64u * U) {
+          unsigned long long syn[U], r[U];
+          bool ok[U];
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
+          for (int u = 0; u < U; u++) {
+            uint32_t e = o + 64u * u + lane;
+            ok[u] = e < c;
+            syn[u] = st + e;
+            r[u] = ok[u] ? A.rec[syn[u]] : 0ull;
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++) f(ok[u], syn[u], r[u], h, mu);
+        }
+      }
+      continue;
+    }
+    // Short segments: the tile's events are spread over the lanes; each
+    // lane finds its pair by a binary lift over the exclusive scan.
+    for (uint32_t e0 = 0; e0 < E; e0 += 64u * U) {
+      unsigned long long syn[U], r[U];
+      float hj[U];
+      uint32_t mj[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
         uint32_t e = e0 + 64u * u + lane;
         // owner lane j < pt: last lane with pre[j] <= e
         int j = 0;
@@ -1154,20 +1193,28 @@ __device__ __forceinline__ unsigned long long deliver_events(
         r[u] = ok[u] ? A.rec[syn[u]] : 0ull;
       }
 #pragma unroll
-      for (int u = 0; u < 2; u++) f(ok[u], syn[u], r[u], hj[u], mj[u]);
+      for (int u = 0; u < U; u++) f(ok[u], syn[u], r[u], hj[u], mj[u]);
     }
   }
   return n_events;
+}
+
+// Dynamic LDS of the delivery kernels: s_pre[D + 1], then (binned
+// phases) one counter per bin.
+extern __shared__ uint32_t s_dyn[];
+
+inline size_t delivery_lds_bytes(int D, int nb) {
+  return sizeof(uint32_t) * ((size_t)D + 1 + (size_t)nb);
 }
 
 template <bool PLASTIC>
 __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
                                               uint32_t vblock,
                                               uint32_t vgrid) {
-  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
+  uint32_t *s_pre = s_dyn;
   __shared__ uint32_t s_part[kBlock];
   const uint32_t total = delivery_prologue(A, vblock, s_pre, s_part);
-  unsigned long long n_events = deliver_events(
+  unsigned long long n_events = deliver_events<2>(
       A, vblock, vgrid, s_pre, total,
       [&](bool ok, unsigned long long syn, unsigned long long r, float hj,
           uint32_t mj) {
@@ -1210,21 +1257,47 @@ __global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
 // accumulators run at a fixed rate per device whatever the address
 // pattern, so when a step carries many events per accumulator entry the
 // events are first grouped by accumulator range (bin of kBinLen entries):
-//   1 count    per (bin, block) event counts, LDS histogram
-//   2 scan     per bin, exclusive offsets over the blocks, bin totals
+//   1 count    per (sub-bin, block) event counts, LDS histogram
+//   2 scan     per sub-bin, exclusive offsets over the blocks, totals
 //   3 scatter  the same enumeration again; each event is written as
 //              (index within bin, value) at its bin's cursor; block 0
 //              also writes the bin and chunk starts.  If the step has more
 //              events than the buffer holds, this phase adds them with
 //              atomics instead and the later phases do nothing.
-//   4 reduce   one block per chunk of <= kBinChunk events of one bin sums
-//              them in LDS; a bin of one chunk is added to the
+//   4 reduce   one block per chunk of events of one bin sums them in LDS
+//              (chunk size chosen from the step's event count, see
+//              bin_chunk_size); a bin of one chunk is added to the
 //              accumulators directly, otherwise the partial sums are
 //              written out
 //   5 sum      bins of several chunks: accumulators += sum of partials.
 // Count and scatter use the same grid and tile settings, so every block
 // enumerates the same events in both.
 // ---------------------------------------------------------------------
+
+constexpr int kReduceBlock = 1024;
+
+// Float add into LDS by compare-and-swap.  On gfx950 the LDS float atomic
+// add runs at about a twelfth of the rate of the integer atomics and of a
+// plain read-modify-write; a CAS loop whose first attempt nearly always
+// succeeds (random addresses in a table of kBinLen entries) is faster.
+__device__ __forceinline__ void lds_add_f32(float *p, float v) {
+  uint32_t *u = reinterpret_cast<uint32_t *>(p);
+  uint32_t old = *u;
+  for (;;) {
+    uint32_t nw = __float_as_uint(__uint_as_float(old) + v);
+    uint32_t prev = atomicCAS(u, old, nw);
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+__device__ __forceinline__ uint32_t bin_chunk_size(unsigned long long n_ev) {
+  unsigned long long ch = (n_ev + kReduceChunks - 1) / kReduceChunks;
+  ch = (ch + 1023ull) & ~1023ull;
+  if (ch < (unsigned long long)kBinChunkMin) ch = kBinChunkMin;
+  if (ch > 0x40000000ull) ch = 0x40000000ull;
+  return (uint32_t)ch;
+}
 
 // Exclusive scan of val(0..n-1) by one block: out[b] = sum of val(b') for
 // b' < b.  Returns the 64-bit total.  s_part holds kBlock entries.
@@ -1262,26 +1335,26 @@ __device__ __forceinline__ unsigned long long block_scan_excl(
 
 __global__ __launch_bounds__(kBlock) void bin_count_kernel(DeliveryArgs A,
                                                            BinArgs B) {
-  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
+  uint32_t *s_pre = s_dyn;
+  uint32_t *s_h = s_dyn + A.D + 1;
   __shared__ uint32_t s_part[kBlock];
-  __shared__ uint32_t s_h[kMaxBins];
-  for (int b = threadIdx.x; b < B.nb; b += kBlock) s_h[b] = 0u;
+  for (int b = threadIdx.x; b < B.nsub; b += kBlock) s_h[b] = 0u;
   const uint32_t total = delivery_prologue(A, blockIdx.x, s_pre, s_part);
-  deliver_events(A, blockIdx.x, gridDim.x, s_pre, total,
+  deliver_events<4>(A, blockIdx.x, gridDim.x, s_pre, total,
                  [&](bool ok, unsigned long long, unsigned long long r, float,
                      uint32_t) {
                    if (!ok) return;
                    uint32_t tp = (uint32_t)r;
                    long long idx = (long long)(tp >> kPortShift) * A.n_total +
                                    (tp & kTargetMask);
-                   atomicAdd(&s_h[idx >> kBinShift], 1u);
+                   atomicAdd(&s_h[idx >> B.sub_shift], 1u);
                  });
   __syncthreads();
-  for (int b = threadIdx.x; b < B.nb; b += kBlock)
+  for (int b = threadIdx.x; b < B.nsub; b += kBlock)
     B.hist[(size_t)b * B.grid + blockIdx.x] = s_h[b];
 }
 
-// one block per bin
+// one block per sub-bin
 __global__ __launch_bounds__(kBlock) void bin_scan_kernel(BinArgs B) {
   __shared__ unsigned long long s_part[kBlock];
   uint32_t *h = B.hist + (size_t)blockIdx.x * B.grid;
@@ -1294,32 +1367,43 @@ __global__ __launch_bounds__(kBlock) void bin_scan_kernel(BinArgs B) {
 
 __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A,
                                                              BinArgs B) {
-  __shared__ uint32_t s_pre[kMaxDelaySlots + 1];
+  uint32_t *s_pre = s_dyn;
+  uint32_t *s_cur = s_dyn + A.D + 1;
   __shared__ uint32_t s_part[kBlock];
   __shared__ unsigned long long s_part64[kBlock];
-  __shared__ uint32_t s_cur[kMaxBins];
   const unsigned long long n_ev = block_scan_excl(
-      B.nb, [&](int b) -> unsigned long long { return B.tot[b]; }, s_cur,
+      B.nsub, [&](int b) -> unsigned long long { return B.tot[b]; }, s_cur,
       s_part64);
   const bool binned = n_ev <= (unsigned long long)B.ev_cap;
   if (blockIdx.x == 0) {
-    // bin and chunk starts for the reduce and sum phases
+    // bin and chunk starts for the reduce and sum phases; the chunk size
+    // gives about kReduceChunks chunks in all (at least kBinChunkMin
+    // events each), so the reduction fills the device whatever the step's
+    // event count
+    const uint32_t ch = bin_chunk_size(n_ev);
+    const int spb = kBinLen >> B.sub_shift;  // sub-bins per bin
+    auto start = [&](int b) -> unsigned long long {
+      return b * spb < B.nsub ? s_cur[b * spb] : n_ev;
+    };
     for (int b = threadIdx.x; b < B.nb; b += kBlock)
-      B.bstart[b] = binned ? s_cur[b] : 0u;
-    if (threadIdx.x == 0) B.bstart[B.nb] = binned ? (uint32_t)n_ev : 0u;
+      B.bstart[b] = binned ? (uint32_t)start(b) : 0u;
+    if (threadIdx.x == 0) {
+      B.bstart[B.nb] = binned ? (uint32_t)n_ev : 0u;
+      B.meta[0] = ch;
+    }
     unsigned long long nc = block_scan_excl(
         B.nb,
         [&](int b) -> unsigned long long {
-          return binned ? (B.tot[b] + kBinChunk - 1u) / kBinChunk : 0u;
+          return binned ? (start(b + 1) - start(b) + ch - 1u) / ch : 0u;
         },
         B.cstart, s_part64);
     if (threadIdx.x == 0) B.cstart[B.nb] = (uint32_t)nc;
   }
-  for (int b = threadIdx.x; b < B.nb; b += kBlock)
+  for (int b = threadIdx.x; b < B.nsub; b += kBlock)
     s_cur[b] += B.hist[(size_t)b * B.grid + blockIdx.x];
   // (the prologue's barriers order these LDS writes before their use)
   const uint32_t total = delivery_prologue(A, blockIdx.x, s_pre, s_part);
-  unsigned long long n_events = deliver_events(
+  unsigned long long n_events = deliver_events<4>(
       A, blockIdx.x, gridDim.x, s_pre, total,
       [&](bool ok, unsigned long long, unsigned long long r, float hj,
           uint32_t) {
@@ -1330,7 +1414,7 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A,
                         (tp & kTargetMask);
         float v = hj * w;
         if (binned) {
-          uint32_t pos = atomicAdd(&s_cur[idx >> kBinShift], 1u);
+          uint32_t pos = atomicAdd(&s_cur[idx >> B.sub_shift], 1u);
           B.ev[pos] = ((unsigned long long)__float_as_uint(v) << 32) |
                       (uint32_t)(idx & (kBinLen - 1));
         } else {
@@ -1342,10 +1426,11 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A,
   }
 }
 
-__global__ __launch_bounds__(kBlock) void bin_reduce_kernel(BinArgs B,
-                                                            float *acc) {
+__global__ __launch_bounds__(kReduceBlock) void bin_reduce_kernel(BinArgs B,
+                                                                  float *acc) {
   __shared__ float s_a[kBinLen];
   const uint32_t n_chunk = B.cstart[B.nb];
+  const uint32_t ch = B.meta[0];
   for (uint32_t w = blockIdx.x; w < n_chunk; w += gridDim.x) {
     // bin of chunk w: last b with cstart[b] <= w (it has >= 1 chunk)
     int lo = 0, hi = B.nb;  // cstart[lo] <= w < cstart[hi]
@@ -1357,36 +1442,36 @@ __global__ __launch_bounds__(kBlock) void bin_reduce_kernel(BinArgs B,
     const uint32_t k = w - B.cstart[b];
     const uint32_t nc = B.cstart[b + 1] - B.cstart[b];
     const uint32_t e_end = B.bstart[b + 1];
-    const uint32_t e0 = B.bstart[b] + k * (uint32_t)kBinChunk;
-    const uint32_t e1 = (e_end - e0 < (uint32_t)kBinChunk) ? e_end
-                                                         : e0 + kBinChunk;
-    for (int i = threadIdx.x; i < kBinLen; i += kBlock) s_a[i] = 0.0f;
+    const uint32_t e0 = B.bstart[b] + k * ch;
+    const uint32_t e1 = (e_end - e0 < ch) ? e_end : e0 + ch;
+    for (int i = threadIdx.x; i < kBinLen; i += kReduceBlock) s_a[i] = 0.0f;
     __syncthreads();
     uint32_t e = e0 + threadIdx.x;
-    for (; e + 3u * kBlock < e1; e += 4u * kBlock) {
+    for (; e + 3u * kReduceBlock < e1; e += 4u * kReduceBlock) {
       unsigned long long v[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) v[u] = B.ev[e + (uint32_t)u * kBlock];
+      for (int u = 0; u < 4; u++) v[u] = B.ev[e + (uint32_t)u * kReduceBlock];
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        atomicAdd(&s_a[(uint32_t)v[u]], __uint_as_float((uint32_t)(v[u] >> 32)));
+        lds_add_f32(&s_a[(uint32_t)v[u]],
+                    __uint_as_float((uint32_t)(v[u] >> 32)));
     }
-    for (; e < e1; e += kBlock) {
+    for (; e < e1; e += kReduceBlock) {
       unsigned long long v = B.ev[e];
-      atomicAdd(&s_a[(uint32_t)v], __uint_as_float((uint32_t)(v >> 32)));
+      lds_add_f32(&s_a[(uint32_t)v], __uint_as_float((uint32_t)(v >> 32)));
     }
     __syncthreads();
     const long long a0 = (long long)b * kBinLen;
     const long long lim_ll = B.acc_n - a0;
     const int lim = lim_ll < kBinLen ? (int)lim_ll : kBinLen;
     if (nc == 1) {
-      for (int i = threadIdx.x; i < lim; i += kBlock) {
+      for (int i = threadIdx.x; i < lim; i += kReduceBlock) {
         float x = s_a[i];
         if (x != 0.0f) acc[a0 + i] += x;
       }
     } else {
       float *pp = B.part + (size_t)w * kBinLen;
-      for (int i = threadIdx.x; i < kBinLen; i += kBlock) pp[i] = s_a[i];
+      for (int i = threadIdx.x; i < kBinLen; i += kReduceBlock) pp[i] = s_a[i];
     }
     __syncthreads();
   }
@@ -1512,12 +1597,13 @@ template <int P>
 void launch_aeif_pass2_p(const AeifArgs &a, const DeliveryArgs *d,
                          int d_grid, hipStream_t s) {
   int grid2 = a.pass2_grid > 0 ? a.pass2_grid : 256;
+  const size_t lds = d ? delivery_lds_bytes(d->D, 0) : 0;
   if (d && a.uniform)
     hipLaunchKernelGGL((aeif_tail_kernel<P, true>), dim3(grid2 + d_grid),
-                       dim3(kBlock), 0, s, a, *d, (uint32_t)grid2);
+                       dim3(kBlock), lds, s, a, *d, (uint32_t)grid2);
   else if (d)
     hipLaunchKernelGGL((aeif_tail_kernel<P, false>), dim3(grid2 + d_grid),
-                       dim3(kBlock), 0, s, a, *d, (uint32_t)grid2);
+                       dim3(kBlock), lds, s, a, *d, (uint32_t)grid2);
   else if (a.uniform)
     hipLaunchKernelGGL((aeif_pass2_kernel<P, true>), dim3(grid2), dim3(kBlock),
                        0, s, a);
@@ -1587,23 +1673,26 @@ void LaunchSpikeGen(const SpikeGenArgs &a, void *stream) {
 }
 
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream) {
+  const size_t lds = delivery_lds_bytes(a.D, 0);
   if (a.syn_group)
-    hipLaunchKernelGGL(delivery_kernel<true>, dim3(grid), dim3(kBlock), 0,
+    hipLaunchKernelGGL(delivery_kernel<true>, dim3(grid), dim3(kBlock), lds,
                        (hipStream_t)stream, a);
   else
-    hipLaunchKernelGGL(delivery_kernel<false>, dim3(grid), dim3(kBlock), 0,
+    hipLaunchKernelGGL(delivery_kernel<false>, dim3(grid), dim3(kBlock), lds,
                        (hipStream_t)stream, a);
 }
 
 void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,
                           int reduce_grid, void *stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(bin_count_kernel, dim3(b.grid), dim3(kBlock), 0, s, a, b);
-  hipLaunchKernelGGL(bin_scan_kernel, dim3(b.nb), dim3(kBlock), 0, s, b);
-  hipLaunchKernelGGL(bin_scatter_kernel, dim3(b.grid), dim3(kBlock), 0, s, a,
+  const size_t lds = delivery_lds_bytes(a.D, b.nsub);
+  hipLaunchKernelGGL(bin_count_kernel, dim3(b.grid), dim3(kBlock), lds, s, a,
                      b);
-  hipLaunchKernelGGL(bin_reduce_kernel, dim3(reduce_grid), dim3(kBlock), 0, s,
-                     b, a.acc);
+  hipLaunchKernelGGL(bin_scan_kernel, dim3(b.nsub), dim3(kBlock), 0, s, b);
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(b.grid), dim3(kBlock), lds, s, a,
+                     b);
+  hipLaunchKernelGGL(bin_reduce_kernel, dim3(reduce_grid), dim3(kReduceBlock),
+                     0, s, b, a.acc);
   long long g = (b.acc_n + kBlock - 1) / kBlock;
   if (g > 2048) g = 2048;
   hipLaunchKernelGGL(bin_sum_kernel, dim3((unsigned)g), dim3(kBlock), 0, s, b,

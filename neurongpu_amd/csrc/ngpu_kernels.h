@@ -184,19 +184,30 @@ struct DeliveryArgs {
 // the accumulators once per entry instead of once per event.
 constexpr int kBinShift = 13;
 constexpr int kBinLen = 1 << kBinShift;   // accumulator entries per bin
-constexpr int kMaxBins = 8192;            // bins a block can count in LDS
-constexpr int kBinChunk = 65536;          // events summed by one block
+constexpr int kMaxBins = 8192;            // sub-bins a block can count in LDS
+constexpr int kSubBinsTarget = 64;        // preferred largest sub-bin count
+constexpr int kReduceChunks = 512;        // target chunk count of a step
+constexpr int kBinChunkMin = 4096;        // fewest events per chunk
 
 struct BinArgs {
-  uint32_t *hist;           // [nb][grid] counts, then offsets within the bin
-  uint32_t *tot;            // [nb] events per bin
+  // The count and scatter phases group events by sub-bin (2^sub_shift
+  // accumulator entries, at most kBinLen), fine enough that the lanes of a
+  // wavefront rarely hit the same LDS counter; a bin is a run of
+  // kBinLen >> sub_shift consecutive sub-bins, contiguous in the buffer.
+  uint32_t *hist;           // [nsub][grid] counts, then offsets within the
+                            // sub-bin
+  uint32_t *tot;            // [nsub] events per sub-bin
   uint32_t *bstart;         // [nb+1] first event of each bin
   uint32_t *cstart;         // [nb+1] first chunk of each bin
   unsigned long long *ev;   // [ev_cap] (index within bin, value bits)
-  float *part;              // [chunk_cap][kBinLen] partial sums
+  float *part;              // [chunk_cap][kBinLen] partial sums,
+                            // chunk_cap = kReduceChunks + nb
+  uint32_t *meta;           // [1] chunk size of the step
   long long ev_cap;         // above this many events the step falls back
                             // to atomic delivery
   int nb;                   // bins: ceil(acc_n / kBinLen)
+  int nsub;                 // sub-bins: ceil(acc_n >> sub_shift)
+  int sub_shift;
   int grid;                 // blocks of the count and scatter phases
   long long acc_n;          // accumulator entries (ports x nodes)
 };

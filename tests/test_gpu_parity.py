@@ -256,9 +256,10 @@ def test_network_statistics_vs_fp64(ngpu):
 
 
 @pytest.mark.parametrize("settings", [
-    {},
+    {"delivery_mode": 0},
     {"delivery_mode": 1},
     {"delivery_mode": 1, "delivery_bin_capacity": 50000},
+    {"delivery_mode": 2, "delivery_auto_events": 100000},
 ])
 def test_c2_iaf_network_bit_exact(ngpu, settings):
     """c2-shaped iaf_psc_exp network at 10 % size (7717 neurons, 5e6
@@ -266,8 +267,9 @@ def test_c2_iaf_network_bit_exact(ngpu, settings):
     every synaptic sum is exact: spike trains equal the FP32 oracle's spike
     for spike over 100 ms.  Run with per-event atomics, with binned
     delivery (about 1e5 events per bin per step, so bins span several
-    reduction chunks) and with a binned-event buffer too small for the
-    busy steps (those fall back to atomics)."""
+    reduction chunks), with a binned-event buffer too small for the
+    busy steps (those fall back to atomics) and in the automatic mode with
+    a threshold the network crosses after its first few milliseconds."""
     from neurongpu_amd import workloads as wl
     log("case c2_scaled", settings)
     spec = wl.c2_spec(scale=0.1, integer_weights=True)

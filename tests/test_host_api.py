@@ -337,15 +337,16 @@ def test_launch_settings(ngpu):
     and range checks (reported like the reference's parameter errors)."""
     defaults = {"rk5_two_pass": 1, "rk5_budget": 1, "rk5_pass2_grid": 1024,
                 "fuse_delivery": 1, "delivery_grid": 2048,
-                "delivery_tile_max": 64, "delivery_mode": 0,
-                "delivery_bin_capacity": 1 << 27}
+                "delivery_tile_max": 64, "delivery_mode": 2,
+                "delivery_bin_capacity": 1 << 27,
+                "delivery_auto_events": 500000}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)
     assert ngpu.GetKernelStatus("delivery_mode") == 1
     ngpu.SetKernelStatus("delivery_bin_capacity", 1000)
     assert ngpu.GetKernelStatus("delivery_bin_capacity") == 1000
-    for k, bad in (("delivery_mode", 2), ("delivery_tile_max", 3),
+    for k, bad in (("delivery_mode", 3), ("delivery_tile_max", 3),
                    ("delivery_grid", 0), ("rk5_budget", 0),
                    ("delivery_bin_capacity", 0)):
         with pytest.raises(ValueError):

@@ -6,7 +6,9 @@ stretches of the running simulation.  The settings are:
 * the delivery mode (one float atomic per synaptic event, or events
   binned by accumulator range and summed in LDS first), the delivery grid
   and the largest number of (spike, delay) pairs per wave tile;
-* whether the delivery shares the second RK5 pass's launch.
+* whether the delivery shares the second RK5 pass's launch;
+* whether Simulate replays captured hipGraphs of 16 steps or launches
+  every step from the host.
 
 None of these changes results: each neuron's arithmetic and each synapse's
 delivery are the same under every setting. The tests check this spike for
@@ -22,6 +24,7 @@ CANDIDATES = {
     "delivery_grid": [1024, 2048, 4096],
     "delivery_tile_max": [8, 16, 64],
     "fuse_delivery": [1, 0],
+    "graph_steps": [0, 16],
 }
 
 

@@ -93,7 +93,6 @@ struct Group {
   DBuf<float> d_rec_times;
   DBuf<unsigned char> d_strag;      // resume records of the two-pass update
   DBuf<unsigned int> d_strag_n;     // [2][64] sub-list lengths
-  unsigned long long n_strag_launch = 0;
   // synchronisation between host mirror and device
   bool host_newer = true;   // host arrays changed since last upload
   bool dev_newer = false;   // device advanced since last download
@@ -215,6 +214,27 @@ class Engine {
   long long snap_gs = -1, prev_snap_gs = -1;
   unsigned long long prev_snap_total = 0;
   bool AutoBinned();
+  // hipGraph replay of whole steps: graph_steps steps per captured graph
+  // (0: never, the default: on MI355X with ROCm 7.2 a replayed step of
+  // the 1M-neuron network took longer than the same launches issued one by
+  // one), one graph per delivery variant (atomic, binned); the captured
+  // launches read their step values from d_clock
+  int graph_steps = 0;
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    int steps = 0;
+  };
+  StepGraph graphs[2];
+  DBuf<StepClock> d_clock;
+  bool clock_mode = false;    // launches being captured take d_clock
+  int clock_slot = 0;         // clock of the step being captured
+  bool clock_synced = false;  // d_clock[0] holds the host's next step
+  const StepClock *StepClockArg() const {
+    return clock_mode ? d_clock.p + clock_slot : nullptr;
+  }
+  StepClock *StepClockNext() const { return d_clock.p + (clock_slot ^ 1); }
+  void DropGraphs();
+  StepClock HostClock() const;  // values of the next step
   DBuf<uint32_t> d_bin_hist, d_bin_tot, d_bin_bstart, d_bin_cstart,
       d_bin_meta;
   DBuf<unsigned long long> d_bin_ev;

@@ -41,9 +41,49 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Step-dependent values of a node-update launch: from the arguments, or
+// from the step clock when the launch has one (captured steps).
+struct StepNow {
+  uint32_t *ring_src;   // ring slot of this step
+  float *ring_h;
+  uint32_t *ring_cnt;
+  uint32_t *ring_n;
+  long long T;
+  double t;             // neural time at the end of the step
+  long long pois_T;     // pending Poisson draw folded at the start
+  double pois_t;
+};
+
+__device__ __forceinline__ StepNow step_now(const SpikeOut &o, const FoldIn *f,
+                                            const StepClock *c) {
+  StepNow s;
+  if (c) {
+    const long long slot = c->gs % o.R;
+    s.ring_src = o.ring_src0 + slot * o.ring_cap;
+    s.ring_h = o.ring_h0 + slot * o.ring_cap;
+    s.ring_cnt = o.ring_cnt0 + slot * o.ring_cap;
+    s.ring_n = o.ring_n0 + slot;
+    s.T = c->T;
+    s.t = c->t;
+    s.pois_T = c->pois_T;
+    s.pois_t = c->pois_t;
+  } else {
+    s.ring_src = o.ring_src;
+    s.ring_h = o.ring_h;
+    s.ring_cnt = o.ring_cnt;
+    s.ring_n = o.ring_n;
+    s.T = o.T;
+    s.t = o.t;
+    s.pois_T = f ? f->pois_T : -1;
+    s.pois_t = f ? f->pois_t : 0.0;
+  }
+  return s;
+}
+
 // Wavefront-aggregated append of this lane's spikes to the ring slot of the
 // step, plus the per-node bookkeeping of PushSpike.
-__device__ __forceinline__ void emit_spikes(const SpikeOut &o, bool active,
+__device__ __forceinline__ void emit_spikes(const SpikeOut &o,
+                                            const StepNow &sn, bool active,
                                             int i, int node, int nsp,
                                             float height) {
   bool spk = active && nsp > 0;
@@ -51,22 +91,23 @@ __device__ __forceinline__ void emit_spikes(const SpikeOut &o, bool active,
   if (mask == 0ull) return;
   uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
   uint32_t base = 0;
-  if (lane_id() == leader) base = atomicAdd(o.ring_n, (uint32_t)__popcll(mask));
+  if (lane_id() == leader)
+    base = atomicAdd(sn.ring_n, (uint32_t)__popcll(mask));
   base = __shfl(base, (int)leader);
   if (!spk) return;
   uint32_t pos = base + lanes_below(mask);
-  o.ring_src[pos] = (uint32_t)node;
-  o.ring_h[pos] = height;
-  o.ring_cnt[pos] = (uint32_t)nsp;
-  o.last_spike[node] = o.T;
+  sn.ring_src[pos] = (uint32_t)node;
+  sn.ring_h[pos] = height;
+  sn.ring_cnt[pos] = (uint32_t)nsp;
+  o.last_spike[node] = sn.T;
   o.last_height[node] = height / (float)nsp;
-  if (o.den_idx[node] == 0) o.last_rev[node] = o.T;
+  if (o.den_idx[node] == 0) o.last_rev[node] = sn.T;
   if (o.spike_count) o.spike_count[i] += nsp;
   if (o.rec_times) {
     int nr = o.n_rec_times[i];
     for (int s = 0; s < nsp; s++) {
       if (nr >= o.max_rec_times - 1) break;
-      o.rec_times[(long long)i * o.max_rec_times + nr] = (float)o.t;
+      o.rec_times[(long long)i * o.max_rec_times + nr] = (float)sn.t;
       nr++;
     }
     o.n_rec_times[i] = nr;
@@ -76,7 +117,8 @@ __device__ __forceinline__ void emit_spikes(const SpikeOut &o, bool active,
 // Port input of the previous step: delivery accumulator of every port plus
 // the Poisson draws of the node's direct connections (canonical order).
 template <int P>
-__device__ __forceinline__ void gather_input(const FoldIn &f, int node,
+__device__ __forceinline__ void gather_input(const FoldIn &f,
+                                             const StepNow &sn, int node,
                                              float (&a)[P]) {
 #pragma unroll
   for (int p = 0; p < P; p++) {
@@ -84,13 +126,13 @@ __device__ __forceinline__ void gather_input(const FoldIn &f, int node,
     a[p] = *ap;
     *ap = 0.0f;
   }
-  if (f.pois_T < 0 || !f.dir.active) return;
+  if (sn.pois_T < 0 || !f.dir.active) return;
   const DirectIn &d = f.dir;
   uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
-  uint32_t Tlo = (uint32_t)f.pois_T, Thi = (uint32_t)((uint64_t)f.pois_T >> 32);
+  uint32_t Tlo = (uint32_t)sn.pois_T, Thi = (uint32_t)((uint64_t)sn.pois_T >> 32);
   for (uint32_t j = j0; j < j1; j++) {
     uint32_t g = d.gen[j];
-    double t_rel = f.pois_t - (double)d.gen_origin[g] - (double)d.delay[j];
+    double t_rel = sn.pois_t - (double)d.gen_origin[g] - (double)d.delay[j];
     if (!(t_rel >= (double)d.gen_start[g] && t_rel <= (double)d.gen_stop[g]))
       continue;
     uint32_t ti = d.gen_table[g];
@@ -281,6 +323,7 @@ __device__ __forceinline__ bool aeif_rk5_step(const AeifLocal<P> &p, float refr,
 
 template <int P>
 __global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
+  const StepNow sn = step_now(A.out, &A.fold, A.clock);
   int i = blockIdx.x * kBlock + threadIdx.x;
   bool active = i < A.n;
   int ii = active ? i : 0;
@@ -320,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
     }
     // fold the previous step's port input into g1
     float in[P];
-    gather_input<P>(A.fold, node, in);
+    gather_input<P>(A.fold, sn, node, in);
 #pragma unroll
     for (int q = 0; q < P; q++)
       y[3 + 2 * q] = __builtin_fmaf(in[q], p.g0[q], y[3 + 2 * q]);
@@ -328,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
       double x = A.x[i];
       float h = A.h[i];
       float refr = A.refr[i];
-      double x1 = A.x1;
+      double x1 = A.clock ? sn.t : A.x1;
       int n_att = 0;
       bool end = false;
       while (!end) {
@@ -373,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
     }
   }
   if (!A.do_step) return;
-  emit_spikes(A.out, active, ii, node, nsp, (float)nsp);
+  emit_spikes(A.out, sn, active, ii, node, nsp, (float)nsp);
   // statistics: accepted / rejected RK5 attempts
   int sa = n_acc, sr = n_rej;
 #pragma unroll
@@ -565,12 +608,12 @@ static_assert(sizeof(StragRec) == kStragRecBytes, "resume record size");
 // the time step.
 template <int P>
 __device__ __forceinline__ bool aeif_attempt_and_update(
-    const AeifArgs &A, const AeifLocal<P> &p, float (&y)[2 + 2 * P],
-    const float (&k1)[2 + 2 * P], const float (&inv_scal)[2 + 2 * P],
-    double &x, float &h, float &refr, int &nsp, int &n_att, bool &need_k1,
-    int &n_acc, int &n_rej, float &err_out, float &hh_out) {
+    const AeifArgs &A, const double x1, const AeifLocal<P> &p,
+    float (&y)[2 + 2 * P], const float (&k1)[2 + 2 * P],
+    const float (&inv_scal)[2 + 2 * P], double &x, float &h, float &refr,
+    int &nsp, int &n_att, bool &need_k1, int &n_acc, int &n_rej,
+    float &err_out, float &hh_out) {
   constexpr int N = 2 + 2 * P;
-  const double x1 = A.x1;
   float yn[N];
   float h_max = (float)(x1 - x);
   float hh = h;
@@ -658,6 +701,11 @@ __device__ __forceinline__ void aeif_count(const AeifArgs &A, int n_acc,
   }
 }
 
+// Sub-list length set of the step (even / odd gs) and the other set.
+__device__ __forceinline__ long long aeif_gs(const AeifArgs &A) {
+  return A.clock ? A.clock->gs : A.gs;
+}
+
 // First pass: one neuron per lane, coalesced, at most A.budget RK5
 // attempts.  Neurons that need more (those that spike or sit near
 // threshold, which would otherwise hold their 63 wave neighbours idle)
@@ -665,10 +713,15 @@ __device__ __forceinline__ void aeif_count(const AeifArgs &A, int n_acc,
 template <int P, bool UNI>
 __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
   constexpr int N = 2 + 2 * P;
+  const StepNow sn = step_now(A.out, &A.fold, A.clock);
+  const double x1 = A.clock ? sn.t : A.x1;
+  const long long gs = aeif_gs(A);
+  unsigned int *strag_n = A.strag_n2 + kStragLists * (gs & 1);
+  unsigned int *strag_n_next = A.strag_n2 + kStragLists * ((gs + 1) & 1);
   __shared__ uint32_t s_n, s_base;
   if (threadIdx.x == 0) s_n = 0u;
   if (blockIdx.x == 0 && threadIdx.x < kStragLists)
-    A.strag_n_next[threadIdx.x] = 0u;
+    strag_n_next[threadIdx.x] = 0u;
   __syncthreads();
   uint32_t slot = 0xFFFFFFFFu;
   StragRec rec;
@@ -690,7 +743,7 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
       y[3 + 2 * q] = A.g1[q][i];
     }
     float in[P];
-    gather_input<P>(A.fold, node, in);
+    gather_input<P>(A.fold, sn, node, in);
 #pragma unroll
     for (int q = 0; q < P; q++)
       y[3 + 2 * q] = __builtin_fmaf(in[q], p.g0[q], y[3 + 2 * q]);
@@ -707,9 +760,9 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
         aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
         need_k1 = false;
       }
-      if (aeif_attempt_and_update<P>(A, p, y, k1, inv_scal, x, h, refr, nsp,
-                                     n_att, need_k1, n_acc, n_rej, err_last,
-                                     hh_used)) {
+      if (aeif_attempt_and_update<P>(A, x1, p, y, k1, inv_scal, x, h, refr,
+                                     nsp, n_att, need_k1, n_acc, n_rej,
+                                     err_last, hh_used)) {
         finished = true;
         break;
       }
@@ -725,14 +778,14 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
       rec.err_last = err_last;
     }
   }
-  emit_spikes(A.out, active && finished, ii, node, nsp, (float)nsp);
+  emit_spikes(A.out, sn, active && finished, ii, node, nsp, (float)nsp);
   aeif_count<P>(A, n_acc, n_rej);
   // this block's resume records go to sub-list blockIdx % kStragLists,
   // at an offset reserved with one atomic per block
   __syncthreads();
   const uint32_t list = blockIdx.x % kStragLists;
   if (threadIdx.x == 0)
-    s_base = s_n ? atomicAdd(A.strag_n + list, s_n) : 0u;
+    s_base = s_n ? atomicAdd(strag_n + list, s_n) : 0u;
   __syncthreads();
   if (slot != 0xFFFFFFFFu)
     reinterpret_cast<StragRec *>(A.strag)[(size_t)list * A.strag_cap +
@@ -750,6 +803,10 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
                                                 uint32_t vblock,
                                                 uint32_t vgrid) {
   constexpr int N = 2 + 2 * P;
+  const StepNow sn = step_now(A.out, &A.fold, A.clock);
+  const double x1 = A.clock ? sn.t : A.x1;
+  const unsigned int *strag_n =
+      A.strag_n2 + kStragLists * (aeif_gs(A) & 1);
   __shared__ uint32_t s_pre[kStragLists + 1];
   // proposer weights, staged once per block (read at every step start)
   __shared__ __attribute__((aligned(16))) unsigned char s_pk[kPropPkBytes];
@@ -760,7 +817,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       dst[k] = src[k];
   }
   if (threadIdx.x < 64) {  // one wave: exclusive scan of the 64 lengths
-    uint32_t c = A.strag_n[threadIdx.x];
+    uint32_t c = strag_n[threadIdx.x];
     uint32_t incl = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -846,9 +903,9 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
         need_k1 = false;
       }
       if (A.prop_log) prop_features<P>(p, y, h, err_last, 1.0f / A.dt, feat);
-      finished = aeif_attempt_and_update<P>(A, p, y, k1, inv_scal, x, h, refr,
-                                            nsp, n_att, need_k1, n_acc, n_rej,
-                                            err, hh);
+      finished = aeif_attempt_and_update<P>(A, x1, p, y, k1, inv_scal, x, h,
+                                            refr, nsp, n_att, need_k1, n_acc,
+                                            n_rej, err, hh);
       err_last = err;
       attempted = true;
       if (finished) aeif_store_state<P>(A, i, y, x, h, refr);
@@ -875,7 +932,8 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
         }
       }
     }
-    emit_spikes(A.out, finished, finished ? i : 0, A.i0 + (finished ? i : 0),
+    emit_spikes(A.out, sn, finished, finished ? i : 0,
+                A.i0 + (finished ? i : 0),
                 nsp, (float)nsp);
     if (finished) have = false;
   }
@@ -892,6 +950,39 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
                                               uint32_t vblock,
                                               uint32_t vgrid);
 
+// End of a captured step: block 0 of its last launch writes the next
+// step's clock (computed as the host computes it in SimulationStep).
+__device__ __forceinline__ void clock_step(const StepClock *c, StepClock *cn) {
+  if (!cn || blockIdx.x != 0 || threadIdx.x != 0) return;
+  StepClock v = *c;
+  v.pois_T = v.T;
+  v.pois_t = v.t;
+  v.gs += 1;
+  v.it += 1;
+  v.T = v.T_base + v.it + 1;
+  v.t = v.t0 + v.dt * (double)(v.it + 1);
+  *cn = v;
+}
+
+// Arguments of a delivery or STDP launch with the step values of the
+// clock, when the launch has one.
+__device__ __forceinline__ DeliveryArgs clocked(const DeliveryArgs &D) {
+  DeliveryArgs A = D;
+  if (A.clock) {
+    A.gs = A.clock->gs;
+    A.T = A.clock->T;
+  }
+  return A;
+}
+__device__ __forceinline__ RevArgs clocked(const RevArgs &R) {
+  RevArgs A = R;
+  if (A.clock) {
+    A.gs = A.clock->gs;
+    A.T = A.clock->T;
+  }
+  return A;
+}
+
 // Second RK5 pass and the delivery of the same step in one launch: blocks
 // [0, n2) run the second pass, the others the delivery.  The two are
 // independent within a step when no synapse is plastic (the delivery reads
@@ -902,16 +993,18 @@ template <int P, bool UNI>
 __global__ __launch_bounds__(kBlock) void aeif_tail_kernel(AeifArgs A,
                                                            DeliveryArgs D,
                                                            uint32_t n2) {
+  clock_step(D.clock, D.clock_next);
   if (blockIdx.x < n2)
     aeif_pass2_body<P, UNI>(A, blockIdx.x, n2);
   else
-    delivery_body<false>(D, blockIdx.x - n2, gridDim.x - n2);
+    delivery_body<false>(clocked(D), blockIdx.x - n2, gridDim.x - n2);
 }
 
 // ---------------------------------------------------------------------
 // iaf_psc_exp, exact integration (src/iaf_psc_exp.cu:94-119)
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void iaf_kernel(IafArgs A) {
+  const StepNow sn = step_now(A.out, &A.fold, A.clock);
   int i = blockIdx.x * kBlock + threadIdx.x;
   bool active = i < A.n;
   int ii = active ? i : 0;
@@ -919,7 +1012,7 @@ __global__ __launch_bounds__(kBlock) void iaf_kernel(IafArgs A) {
   int nsp = 0;
   if (active) {
     float in[2];
-    gather_input<2>(A.fold, node, in);
+    gather_input<2>(A.fold, sn, node, in);
     float I_ex = A.I_ex[i] + in[0];
     float I_in = A.I_in[i] + in[1];
     if (A.do_step) {
@@ -948,13 +1041,14 @@ __global__ __launch_bounds__(kBlock) void iaf_kernel(IafArgs A) {
     A.I_in[i] = I_in;
   }
   if (!A.do_step) return;
-  emit_spikes(A.out, active, ii, node, nsp, 1.0f);
+  emit_spikes(A.out, sn, active, ii, node, nsp, 1.0f);
 }
 
 // ---------------------------------------------------------------------
 // spike_detector / parrot_neuron
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void relay_kernel(RelayArgs A) {
+  const StepNow sn = step_now(A.out, &A.fold, A.clock);
   int i = blockIdx.x * kBlock + threadIdx.x;
   bool active = i < A.n;
   int ii = active ? i : 0;
@@ -964,10 +1058,10 @@ __global__ __launch_bounds__(kBlock) void relay_kernel(RelayArgs A) {
   if (active) {
     float in[2];
     if (A.parrot) {
-      gather_input<2>(A.fold, node, in);
+      gather_input<2>(A.fold, sn, node, in);
     } else {
       float in1[1];
-      gather_input<1>(A.fold, node, in1);
+      gather_input<1>(A.fold, sn, node, in1);
       in[0] = in1[0];
     }
     float input = A.input[i] + in[0];
@@ -993,13 +1087,15 @@ __global__ __launch_bounds__(kBlock) void relay_kernel(RelayArgs A) {
     A.input[i] = input;
   }
   if (!A.do_step || !A.parrot) return;
-  emit_spikes(A.out, active, ii, node, nsp, height);
+  emit_spikes(A.out, sn, active, ii, node, nsp, height);
 }
 
 // ---------------------------------------------------------------------
 // spike_generator (src/spike_generator.cu:37-55)
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
+  const StepNow sn = step_now(A.out, nullptr, A.clock);
+  const long long it = A.clock ? A.clock->it : A.it;
   int i = blockIdx.x * kBlock + threadIdx.x;
   bool active = i < A.n;
   int ii = active ? i : 0;
@@ -1007,13 +1103,13 @@ __global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
   float height = 0.0f;
   if (active) {
     int k = A.next[i];
-    if (k < A.n_spikes[i] && (long long)A.time_idx[A.first[i] + k] == A.it) {
+    if (k < A.n_spikes[i] && (long long)A.time_idx[A.first[i] + k] == it) {
       nsp = 1;
       height = A.height[A.first[i] + k];
       A.next[i] = k + 1;
     }
   }
-  emit_spikes(A.out, active, ii, A.i0 + ii, nsp, height);
+  emit_spikes(A.out, sn, active, ii, A.i0 + ii, nsp, height);
 }
 
 // ---------------------------------------------------------------------
@@ -1249,7 +1345,8 @@ __device__ __forceinline__ void delivery_body(const DeliveryArgs &A,
 
 template <bool PLASTIC>
 __global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
-  delivery_body<PLASTIC>(A, blockIdx.x, gridDim.x);
+  clock_step(A.clock, A.clock_next);
+  delivery_body<PLASTIC>(clocked(A), blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------
@@ -1333,8 +1430,9 @@ __device__ __forceinline__ unsigned long long block_scan_excl(
   return total;
 }
 
-__global__ __launch_bounds__(kBlock) void bin_count_kernel(DeliveryArgs A,
+__global__ __launch_bounds__(kBlock) void bin_count_kernel(DeliveryArgs A0,
                                                            BinArgs B) {
+  const DeliveryArgs A = clocked(A0);
   uint32_t *s_pre = s_dyn;
   uint32_t *s_h = s_dyn + A.D + 1;
   __shared__ uint32_t s_part[kBlock];
@@ -1365,8 +1463,9 @@ __global__ __launch_bounds__(kBlock) void bin_scan_kernel(BinArgs B) {
     B.tot[blockIdx.x] = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
 }
 
-__global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A,
+__global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
                                                              BinArgs B) {
+  const DeliveryArgs A = clocked(A0);
   uint32_t *s_pre = s_dyn;
   uint32_t *s_cur = s_dyn + A.D + 1;
   __shared__ uint32_t s_part[kBlock];
@@ -1477,8 +1576,9 @@ __global__ __launch_bounds__(kReduceBlock) void bin_reduce_kernel(BinArgs B,
   }
 }
 
-__global__ __launch_bounds__(kBlock) void bin_sum_kernel(BinArgs B,
-                                                         float *acc) {
+__global__ __launch_bounds__(kBlock) void bin_sum_kernel(
+    BinArgs B, float *acc, const StepClock *clock, StepClock *clock_next) {
+  clock_step(clock, clock_next);
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
        i < B.acc_n; i += (long long)gridDim.x * kBlock) {
     const int b = (int)(i >> kBinShift);
@@ -1496,7 +1596,9 @@ __global__ __launch_bounds__(kBlock) void bin_sum_kernel(BinArgs B,
 // reaches its dendrites this step updates its plastic incoming synapses.
 // One wavefront per candidate node; lanes stride over its inputs.
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs A) {
+__global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs A0) {
+  clock_step(A0.clock, A0.clock_next);
+  const RevArgs A = clocked(A0);
   const uint32_t lane = lane_id();
   const uint32_t wave = (uint32_t)(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);
   const uint32_t n_waves = gridDim.x * (kBlock / 64);
@@ -1533,7 +1635,8 @@ __global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs A) {
 // Spikes of nodes with a dendritic delay of k > 0 steps reach the
 // dendrites k steps after emission (src/spike_buffer.cu:276-278,299-301):
 // mark them before this step's delivery reads last_rev.
-__global__ __launch_bounds__(kBlock) void den_rev_kernel(RevArgs A) {
+__global__ __launch_bounds__(kBlock) void den_rev_kernel(RevArgs A0) {
+  const RevArgs A = clocked(A0);
   for (int k = 1; k <= A.max_den; k++) {
     long long s = A.gs - k;
     if (s < 0) break;
@@ -1545,6 +1648,13 @@ __global__ __launch_bounds__(kBlock) void den_rev_kernel(RevArgs A) {
       if (A.den_idx[node] == k) A.last_rev[node] = A.T;
     }
   }
+}
+
+// ---------------------------------------------------------------------
+// Step clock of captured steps
+// ---------------------------------------------------------------------
+__global__ void clock_set_kernel(StepClock *c, StepClock v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *c = v;
 }
 
 __global__ void record_kernel(RecordArgs A) {
@@ -1696,7 +1806,12 @@ void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,
   long long g = (b.acc_n + kBlock - 1) / kBlock;
   if (g > 2048) g = 2048;
   hipLaunchKernelGGL(bin_sum_kernel, dim3((unsigned)g), dim3(kBlock), 0, s, b,
-                     a.acc);
+                     a.acc, a.clock, a.clock_next);
+}
+
+void LaunchClockSet(StepClock *c, const StepClock &v, void *stream) {
+  hipLaunchKernelGGL(clock_set_kernel, dim3(1), dim3(64), 0,
+                     (hipStream_t)stream, c, v);
 }
 
 void LaunchRevStdp(const RevArgs &a, int grid, void *stream) {

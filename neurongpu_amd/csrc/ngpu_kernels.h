@@ -27,6 +27,25 @@ constexpr int kCounterSlots = 1024;   // spread statistic counters over L2
 constexpr uint32_t kPoissonTag = 0x504F4953u;
 constexpr int kStragRecBytes = 24;    // resume record of the two-pass update
 
+// Step clock in device memory.  Kernels launched with a non-null clock
+// read the step-dependent values from it instead of from their arguments,
+// so captured steps (hipGraph) replay unchanged.  Two clocks alternate
+// (even and odd steps of a capture): the last kernel of a step writes the
+// next step's values into the other one (clock_next, from block 0), which
+// no kernel of its own step reads.  The arithmetic is the host's
+// (SimulationStep):
+//   T = T_base + it + 1,  t = t0 + dt * (it + 1)  (double, no fused ops)
+struct StepClock {
+  long long gs;       // global step index (spike ring slot gs % R)
+  long long it;       // step index within the current Simulate call
+  long long T;        // time index of the step
+  double t;           // neural time at the end of the step (ms)
+  long long pois_T;   // time index of the pending Poisson draw (<0 none)
+  double pois_t;      // neural time of that step
+  long long T_base;
+  double t0, dt;
+};
+
 // A parameter that is either uniform over a group (val) or per neuron (ptr).
 struct PField {
   const float *ptr;
@@ -70,6 +89,13 @@ struct SpikeOut {
   float *ring_h;
   uint32_t *ring_cnt;
   uint32_t *ring_n;           // counter of this step's slot
+  // the whole ring, for a clocked launch (slot gs % R of [R][ring_cap])
+  uint32_t *ring_src0;
+  float *ring_h0;
+  uint32_t *ring_cnt0;
+  uint32_t *ring_n0;
+  long long ring_cap;
+  int R;
   long long *last_spike;      // [n_total]
   float *last_height;
   long long *last_rev;
@@ -100,11 +126,14 @@ struct AeifArgs {
   int do_step;                // 0: fold only (end-of-simulation flush)
   // two-pass update: first pass with at most `budget` RK5 attempts per
   // neuron, the rest finished by a second pass over 64 sub-lists of
-  // 16-byte resume records (strag: [64][strag_cap]); strag_n holds the
-  // sub-list lengths of this launch, strag_n_next those of the next launch
-  // (cleared by this one); strag == nullptr selects the single-pass kernel
+  // resume records (strag: [64][strag_cap]); strag_n2 holds two sets of
+  // 64 sub-list lengths, used by the steps of even and odd gs in turn (the
+  // first pass of a step clears the other set for the next step);
+  // strag == nullptr selects the single-pass kernel
   void *strag;
-  unsigned int *strag_n, *strag_n_next;
+  unsigned int *strag_n2;
+  long long gs;
+  const StepClock *clock;     // nullptr: step values from the arguments
   unsigned long long strag_cap;
   int budget;
   int pass2_grid;
@@ -126,6 +155,7 @@ struct IafArgs {
   FoldIn fold;
   SpikeOut out;
   int do_step;
+  const StepClock *clock;
 };
 
 // spike_detector (src/spike_detector.cu:46-68) and parrot_neuron
@@ -139,6 +169,7 @@ struct RelayArgs {
   FoldIn fold;
   SpikeOut out;
   int do_step;
+  const StepClock *clock;
 };
 
 struct SpikeGenArgs {
@@ -149,6 +180,7 @@ struct SpikeGenArgs {
   int *next;
   long long it;                 // step index within this Simulate call
   SpikeOut out;
+  const StepClock *clock;
 };
 
 struct DeliveryArgs {
@@ -176,6 +208,8 @@ struct DeliveryArgs {
   long long T;
   float dt;
   unsigned long long *events;   // [kCounterSlots]
+  const StepClock *clock;       // nullptr: gs and T from the arguments
+  StepClock *clock_next;        // last launch of a captured step only
 };
 
 // Binned delivery (non-plastic networks): the step's synaptic events are
@@ -230,6 +264,8 @@ struct RevArgs {
   long long *last_rev;                // written by the dendritic-delay pass
   long long T;
   float dt;
+  const StepClock *clock;
+  StepClock *clock_next;              // last launch of a captured step only
 };
 
 struct RecordArgs {
@@ -255,6 +291,7 @@ void LaunchIaf(const IafArgs &a, void *stream);
 void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
+void LaunchClockSet(StepClock *c, const StepClock &v, void *stream);
 // Binned delivery: count, scan, scatter, reduce, sum (five launches).
 // Requires a non-plastic network and b.nb <= kMaxBins.
 void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,

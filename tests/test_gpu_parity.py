@@ -191,13 +191,17 @@ def test_stdp_reference_cases(ngpu, case):
     assert abs(float(w) - want) <= 1e-6 * want
 
 
-def test_stdp_network_bit_exact(ngpu):
+@pytest.mark.parametrize("graph_steps", [0, 16])
+def test_stdp_network_bit_exact(ngpu, graph_steps):
     """Plastic network (2000 STDP synapses, dendritic delay 1.5 ms) over
     4 s, past the 16-bit spike-time wrap at step 32768: parrot spike trains
-    and every final plastic weight equal the FP32 oracle's."""
-    log("case stdp_network")
+    and every final plastic weight equal the FP32 oracle's, with steps
+    launched from the host and replayed from captured hipGraphs (steps
+    around the wrap run from the host)."""
+    log("case stdp_network graph_steps", graph_steps)
     spec = nb.stdp_net_spec()
     nb.build_engine(ngpu, spec)
+    ngpu.SetKernelStatus("graph_steps", graph_steps)
     ngpu.ActivateRecSpikeTimes(ngpu.NodeSeq(200, 100), 2000)
     ngpu.Simulate(4000.0)
     eng = []
@@ -378,3 +382,60 @@ def test_launch_shapes_do_not_change_results(ngpu, settings):
     got = sorted(_run_spikes(ngpu, spec, 200.0, settings))
     assert len(ref) > 100
     assert got == ref
+
+
+def _run_spikes_calls(ngpu, spec, calls, t_ms, settings):
+    groups, _ = nb.build_engine(ngpu, spec)
+    for k, v in settings.items():
+        ngpu.SetKernelStatus(k, v)
+    ngpu.ActivateRecSpikeTimes(groups[0], 2000)
+    for _ in range(calls):
+        ngpu.Simulate(t_ms)
+        ngpu.GetHotPathStats()   # host sync and counter read between calls
+    return nb.engine_spikes(ngpu, groups[0])
+
+
+@pytest.mark.parametrize("settings", [
+    {"graph_steps": 16},
+    {"graph_steps": 6},
+    {"graph_steps": 16, "delivery_mode": 1},
+    {"graph_steps": 4, "rk5_two_pass": 0},
+])
+def test_graph_steps_match_eager(ngpu, settings):
+    """Steps replayed from captured hipGraphs (the step clock advanced on
+    the device) spike exactly as steps launched one by one from the host,
+    over four Simulate calls of 50 ms each (each call starts with an eager
+    step; step counts that are not a multiple of the graph length end with
+    eager steps)."""
+    log("case graph_steps", settings)
+    spec = nb.ensemble_spec(_ensemble_instances())
+    ref = sorted(_run_spikes_calls(ngpu, spec, 4, 50.0, {"graph_steps": 0}))
+    got = sorted(_run_spikes_calls(ngpu, spec, 4, 50.0, settings))
+    assert len(ref) > 100
+    assert got == ref
+
+
+def test_graph_steps_stdp_network(ngpu):
+    """Plastic network of the bit-exact STDP test (dyadic weights): parrot
+    spikes and every plastic weight identical with and without captured
+    steps, over two Simulate calls."""
+    log("case graph_steps_stdp")
+    spec = nb.stdp_net_spec()
+    res = []
+    for gsteps in (0, 16):
+        nb.build_engine(ngpu, spec)
+        ngpu.SetKernelStatus("graph_steps", gsteps)
+        ngpu.ActivateRecSpikeTimes(ngpu.NodeSeq(200, 100), 2000)
+        ngpu.Simulate(700.0)
+        ngpu.Simulate(333.3)
+        spikes = []
+        for i in range(200, 300):
+            spikes += [(i, t) for t in ngpu.GetRecSpikeTimes(i)]
+        cid = ngpu.GetConnections(ngpu.NodeSeq(0, 200),
+                                  ngpu.NodeSeq(200, 100))
+        w = sorted((d["source"], d["target"], d["port"], d["weight"])
+                   for d in ngpu.GetStatus(cid))
+        res.append((sorted(spikes), w))
+    assert len(res[0][0]) > 50
+    assert res[1][0] == res[0][0]
+    assert res[1][1] == res[0][1]

@@ -339,15 +339,18 @@ def test_launch_settings(ngpu):
                 "fuse_delivery": 1, "delivery_grid": 2048,
                 "delivery_tile_max": 64, "delivery_mode": 2,
                 "delivery_bin_capacity": 1 << 27,
-                "delivery_auto_events": 500000}
+                "delivery_auto_events": 500000, "graph_steps": 0}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)
     assert ngpu.GetKernelStatus("delivery_mode") == 1
     ngpu.SetKernelStatus("delivery_bin_capacity", 1000)
     assert ngpu.GetKernelStatus("delivery_bin_capacity") == 1000
+    ngpu.SetKernelStatus("graph_steps", 16)
+    assert ngpu.GetKernelStatus("graph_steps") == 16
     for k, bad in (("delivery_mode", 3), ("delivery_tile_max", 3),
                    ("delivery_grid", 0), ("rk5_budget", 0),
-                   ("delivery_bin_capacity", 0)):
+                   ("delivery_bin_capacity", 0), ("graph_steps", -1),
+                   ("graph_steps", 7)):
         with pytest.raises(ValueError):
             ngpu.SetKernelStatus(k, bad)

@@ -28,8 +28,33 @@ enum ModelId {
   kPoissonGenerator,
   kSpikeDetector,
   kParrotNeuron,
+  kAeifPscAlpha,
+  kAeifPscExp,
+  kAeifPscDelta,
+  kIzhikevichCondBeta,
+  kUserM1,          // src/user_m1_kernel.h: aeif_cond_beta under another name
+  kUserM2,          // src/user_m2_kernel.h: the same
   kNumModels
 };
+
+// AdEx conductance models (two-pass RK5 kernels)
+inline bool IsAeifCond(int m) {
+  return m == kAeifCondBeta || m == kAeifCondAlpha || m == kUserM1 ||
+         m == kUserM2;
+}
+inline bool IsCondBeta(int m) {
+  return m == kAeifCondBeta || m == kUserM1 || m == kUserM2;
+}
+// the other RK5 models (rk5_models.hip): their Rk5Kind, or -1
+inline int Rk5KindOf(int m) {
+  switch (m) {
+    case kAeifPscAlpha: return kRk5PscAlpha;
+    case kAeifPscExp: return kRk5PscExp;
+    case kAeifPscDelta: return kRk5PscDelta;
+    case kIzhikevichCondBeta: return kRk5IzhCondBeta;
+    default: return -1;
+  }
+}
 
 struct ModelDesc {
   const char *name;

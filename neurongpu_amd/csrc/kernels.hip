@@ -27,130 +27,11 @@
 #include "ngpu_device.h"
 #include "ngpu_kernels.h"
 #include "proposer.h"
+#include "step_common.h"
 
 namespace ngpu {
 
 namespace {
-
-constexpr int kBlock = 256;
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-// Step-dependent values of a node-update launch: from the arguments, or
-// from the step clock when the launch has one (captured steps).
-struct StepNow {
-  uint32_t *ring_src;   // ring slot of this step
-  float *ring_h;
-  uint32_t *ring_cnt;
-  uint32_t *ring_n;
-  long long T;
-  double t;             // neural time at the end of the step
-  long long pois_T;     // pending Poisson draw folded at the start
-  double pois_t;
-};
-
-__device__ __forceinline__ StepNow step_now(const SpikeOut &o, const FoldIn *f,
-                                            const StepClock *c) {
-  StepNow s;
-  if (c) {
-    const long long slot = c->gs % o.R;
-    s.ring_src = o.ring_src0 + slot * o.ring_cap;
-    s.ring_h = o.ring_h0 + slot * o.ring_cap;
-    s.ring_cnt = o.ring_cnt0 + slot * o.ring_cap;
-    s.ring_n = o.ring_n0 + slot;
-    s.T = c->T;
-    s.t = c->t;
-    s.pois_T = c->pois_T;
-    s.pois_t = c->pois_t;
-  } else {
-    s.ring_src = o.ring_src;
-    s.ring_h = o.ring_h;
-    s.ring_cnt = o.ring_cnt;
-    s.ring_n = o.ring_n;
-    s.T = o.T;
-    s.t = o.t;
-    s.pois_T = f ? f->pois_T : -1;
-    s.pois_t = f ? f->pois_t : 0.0;
-  }
-  return s;
-}
-
-// Wavefront-aggregated append of this lane's spikes to the ring slot of the
-// step, plus the per-node bookkeeping of PushSpike.
-__device__ __forceinline__ void emit_spikes(const SpikeOut &o,
-                                            const StepNow &sn, bool active,
-                                            int i, int node, int nsp,
-                                            float height) {
-  bool spk = active && nsp > 0;
-  unsigned long long mask = __ballot(spk);
-  if (mask == 0ull) return;
-  uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
-  uint32_t base = 0;
-  if (lane_id() == leader)
-    base = atomicAdd(sn.ring_n, (uint32_t)__popcll(mask));
-  base = __shfl(base, (int)leader);
-  if (!spk) return;
-  uint32_t pos = base + lanes_below(mask);
-  sn.ring_src[pos] = (uint32_t)node;
-  sn.ring_h[pos] = height;
-  sn.ring_cnt[pos] = (uint32_t)nsp;
-  o.last_spike[node] = sn.T;
-  o.last_height[node] = height / (float)nsp;
-  if (o.den_idx[node] == 0) o.last_rev[node] = sn.T;
-  if (o.spike_count) o.spike_count[i] += nsp;
-  if (o.rec_times) {
-    int nr = o.n_rec_times[i];
-    for (int s = 0; s < nsp; s++) {
-      if (nr >= o.max_rec_times - 1) break;
-      o.rec_times[(long long)i * o.max_rec_times + nr] = (float)sn.t;
-      nr++;
-    }
-    o.n_rec_times[i] = nr;
-  }
-}
-
-// Port input of the previous step: delivery accumulator of every port plus
-// the Poisson draws of the node's direct connections (canonical order).
-template <int P>
-__device__ __forceinline__ void gather_input(const FoldIn &f,
-                                             const StepNow &sn, int node,
-                                             float (&a)[P]) {
-#pragma unroll
-  for (int p = 0; p < P; p++) {
-    float *ap = f.acc + (long long)p * f.n_total + node;
-    a[p] = *ap;
-    *ap = 0.0f;
-  }
-  if (sn.pois_T < 0 || !f.dir.active) return;
-  const DirectIn &d = f.dir;
-  uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
-  uint32_t Tlo = (uint32_t)sn.pois_T, Thi = (uint32_t)((uint64_t)sn.pois_T >> 32);
-  for (uint32_t j = j0; j < j1; j++) {
-    uint32_t g = d.gen[j];
-    double t_rel = sn.pois_t - (double)d.gen_origin[g] - (double)d.delay[j];
-    if (!(t_rel >= (double)d.gen_start[g] && t_rel <= (double)d.gen_stop[g]))
-      continue;
-    uint32_t ti = d.gen_table[g];
-    if (ti == 0xFFFFFFFFu) continue;
-    const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
-    uint32_t u = dev_philox_w0(d.canon[j], Tlo, Thi, kPoissonTag, d.key0,
-                               d.key1);
-    int k = 0;
-    while (u > tab[k]) k++;
-    if (k > 0) {
-      float add = d.weight[j] * (float)k;
-      int port = d.port[j];
-#pragma unroll
-      for (int p = 0; p < P; p++)
-        if (port == p) a[p] = a[p] + add;
-    }
-  }
-}
 
 // ---------------------------------------------------------------------
 // AdEx conductance neurons (aeif_cond_beta / aeif_cond_alpha), RK5.
@@ -190,8 +71,17 @@ __device__ __forceinline__ void aeif_rhs(const AeifLocal<P> &p, float refr,
   }
 }
 
-// Start of an adaptive Cash-Karp step (src/rk5.h:57-69): first stage and
-// the error scale, which stay fixed over the step's retries.
+// The AdEx RHS as a model of the generic RK5 attempt.
+template <int P>
+struct AeifModel {
+  static constexpr int N = 2 + 2 * P;
+  const AeifLocal<P> &p;
+  __device__ __forceinline__ void rhs(float refr, const float (&y)[N],
+                                      float (&dy)[N]) const {
+    aeif_rhs<P>(p, refr, y, dy);
+  }
+};
+
 template <int P>
 __device__ __forceinline__ void aeif_rk5_begin(const AeifLocal<P> &p,
                                                float refr,
@@ -199,90 +89,17 @@ __device__ __forceinline__ void aeif_rk5_begin(const AeifLocal<P> &p,
                                                float h,
                                                float (&k1)[2 + 2 * P],
                                                float (&inv_scal)[2 + 2 * P]) {
-  constexpr int N = 2 + 2 * P;
-  aeif_rhs<P>(p, refr, y, k1);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float s = fabsf(y[i]) + fabsf(k1[i] * h);
-    s = s + CK::scal_min;
-    inv_scal[i] = 1.0f / s;
-  }
+  rk5_begin(AeifModel<P>{p}, refr, y, h, k1, inv_scal);
 }
 
-// One attempt of the step with size hh (src/rk5.h:70-135): stages 2-6,
-// the fifth-order solution in yn and the scaled error norm (before the
-// division by the tolerance).
 template <int P>
 __device__ __forceinline__ float aeif_rk5_attempt(
     const AeifLocal<P> &p, float refr, const float (&y)[2 + 2 * P],
     const float (&k1)[2 + 2 * P], const float (&inv_scal)[2 + 2 * P],
     float hh, float (&yn)[2 + 2 * P]) {
-  constexpr int N = 2 + 2 * P;
-  float k2[N], k3[N], k4[N], k5[N], k6[N];
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float t = CK::a21 * k1[i];
-    yn[i] = __builtin_fmaf(hh, t, y[i]);
-  }
-  aeif_rhs<P>(p, refr, yn, k2);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float t = CK::a31 * k1[i];
-    t = __builtin_fmaf(CK::a32, k2[i], t);
-    yn[i] = __builtin_fmaf(hh, t, y[i]);
-  }
-  aeif_rhs<P>(p, refr, yn, k3);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float t = CK::a41 * k1[i];
-    t = __builtin_fmaf(CK::a42, k2[i], t);
-    t = __builtin_fmaf(CK::a43, k3[i], t);
-    yn[i] = __builtin_fmaf(hh, t, y[i]);
-  }
-  aeif_rhs<P>(p, refr, yn, k4);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float t = CK::a51 * k1[i];
-    t = __builtin_fmaf(CK::a52, k2[i], t);
-    t = __builtin_fmaf(CK::a53, k3[i], t);
-    t = __builtin_fmaf(CK::a54, k4[i], t);
-    yn[i] = __builtin_fmaf(hh, t, y[i]);
-  }
-  aeif_rhs<P>(p, refr, yn, k5);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float t = CK::a61 * k1[i];
-    t = __builtin_fmaf(CK::a62, k2[i], t);
-    t = __builtin_fmaf(CK::a63, k3[i], t);
-    t = __builtin_fmaf(CK::a64, k4[i], t);
-    t = __builtin_fmaf(CK::a65, k5[i], t);
-    yn[i] = __builtin_fmaf(hh, t, y[i]);
-  }
-  aeif_rhs<P>(p, refr, yn, k6);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float t = CK::a71 * k1[i];
-    t = __builtin_fmaf(CK::a73, k3[i], t);
-    t = __builtin_fmaf(CK::a74, k4[i], t);
-    t = __builtin_fmaf(CK::a76, k6[i], t);
-    yn[i] = __builtin_fmaf(hh, t, y[i]);
-  }
-  float err = 0.0f;
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    float t = CK::e1 * k1[i];
-    t = __builtin_fmaf(CK::e3, k3[i], t);
-    t = __builtin_fmaf(CK::e4, k4[i], t);
-    t = __builtin_fmaf(CK::e5, k5[i], t);
-    t = __builtin_fmaf(CK::e6, k6[i], t);
-    float v = fabsf((hh * t) * inv_scal[i]);
-    err = (err > v) ? err : v;
-  }
-  return err;
+  return rk5_attempt(AeifModel<P>{p}, refr, y, k1, inv_scal, hh, yn);
 }
 
-// One adaptive Cash-Karp step (src/rk5.h:57-147).  Returns false when the
-// attempt cap is reached.
 template <int P>
 __device__ __forceinline__ bool aeif_rk5_step(const AeifLocal<P> &p, float refr,
                                               double &x, float (&y)[2 + 2 * P],
@@ -290,35 +107,8 @@ __device__ __forceinline__ bool aeif_rk5_step(const AeifLocal<P> &p, float refr,
                                               double h_min_acc, float h_max,
                                               float eps, int &n_att,
                                               int &n_acc, int &n_rej) {
-  constexpr int N = 2 + 2 * P;
-  float k1[N], yn[N], inv_scal[N];
-  float err = 0.0f;
-  aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
-  for (;;) {
-    float hh = h;
-    if (hh > h_max) hh = h_max;
-    if (hh < h_min) hh = h_min;
-    h = hh;
-    err = aeif_rk5_attempt<P>(p, refr, y, k1, inv_scal, hh, yn);
-    err = err / eps;
-    n_att++;
-    if (err <= 1.0f || (double)hh <= h_min_acc) break;
-    n_rej++;
-    if (n_att > kMaxAttemptsPerStep) return false;
-    float h_new = (hh * CK::coeff) * dev_pow_dec(err);
-    float h_lo = 0.1f * hh;
-    h = (h_new > h_lo) ? h_new : h_lo;
-  }
-  n_acc++;
-  x += (double)h;
-  if (err > CK::err_min) {
-    h = (h * CK::coeff) * dev_pow_inc(err);
-  } else {
-    h = 5.0f * h;
-  }
-#pragma unroll
-  for (int i = 0; i < N; i++) y[i] = yn[i];
-  return true;
+  return rk5_step(AeifModel<P>{p}, refr, x, y, h, h_min, h_min_acc, h_max,
+                  eps, n_att, n_acc, n_rej);
 }
 
 template <int P>
@@ -688,17 +478,7 @@ __device__ __forceinline__ void aeif_store_state(const AeifArgs &A, int i,
 template <int P>
 __device__ __forceinline__ void aeif_count(const AeifArgs &A, int n_acc,
                                            int n_rej) {
-  int sa = n_acc, sr = n_rej;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    sa += __shfl_xor(sa, off);
-    sr += __shfl_xor(sr, off);
-  }
-  if (lane_id() == 0 && A.cnt && (sa | sr)) {
-    unsigned long long *c = A.cnt + 2 * (blockIdx.x & (kCounterSlots - 1));
-    atomicAdd(c, (unsigned long long)sa);
-    atomicAdd(c + 1, (unsigned long long)sr);
-  }
+  rk5_count(A.cnt, n_acc, n_rej);
 }
 
 // Sub-list length set of the step (even / odd gs) and the other set.

@@ -7,6 +7,13 @@
 //   poisson_generator src/poiss_gen_variables.h, src/poiss_gen.cu:77-95
 //   spike_detector   src/spike_detector.cu:26-38, 72-80
 //   parrot_neuron    src/parrot_neuron.cu:26-46, 75-100
+//   aeif_psc_alpha   src/aeif_psc_alpha_kernel.h:30-105, aeif_psc_alpha.cu:24-48
+//   aeif_psc_exp     src/aeif_psc_exp_kernel.h:30-100, aeif_psc_exp.cu:24-46
+//   aeif_psc_delta   src/aeif_psc_delta_kernel.h:30-95, aeif_psc_delta.cu:24-45
+//   izhikevich_cond_beta  src/izhikevich_cond_beta_kernel.h:30-98,
+//                    izhikevich_cond_beta.cu:24-50
+//   user_m1, user_m2 src/user_m1_kernel.h, src/user_m2_kernel.h (the
+//                    aeif_cond_beta model under other names)
 // The group parameter "rk5_tol" (RK5 error tolerance) has no reference
 // counterpart: the reference hard-codes 1e-6 (src/rk5.cu:55); its default
 // here is the same 1e-6.
@@ -81,6 +88,50 @@ std::vector<ModelDesc> MakeModels() {
   r.scal_param_default = {0.0f, 0.0f};
   r.scal_var_default = {0.0f, 0.0f, 0.0f};
   r.fixed_ports = 2;
+
+  ModelDesc &pa = m[kAeifPscAlpha];
+  pa = b;
+  pa.name = "aeif_psc_alpha";
+  pa.port_var = {"I_syn", "I1_syn"};
+  pa.port_param = {"tau_syn", "I0"};
+  pa.port_param_default = {0.2f, 0.0f};
+
+  ModelDesc &pe = m[kAeifPscExp];
+  pe = b;
+  pe.name = "aeif_psc_exp";
+  pe.port_var = {"I_syn"};
+  pe.port_param = {"tau_syn"};
+  pe.port_param_default = {0.2f};
+  pe.port_var_default = {0.0f};
+
+  ModelDesc &pd = m[kAeifPscDelta];
+  pd = b;
+  pd.name = "aeif_psc_delta";
+  pd.port_var = {};
+  pd.port_param = {};
+  pd.port_param_default = {};
+  pd.port_var_default = {};
+
+  ModelDesc &iz = m[kIzhikevichCondBeta];
+  iz.name = "izhikevich_cond_beta";
+  iz.scal_var = {"V_m", "u"};
+  iz.port_var = {"g", "g1"};
+  iz.scal_param = {"V_th", "a", "b", "c", "d", "I_e", "t_ref",
+                   "refractory_step", "den_delay"};
+  iz.port_param = {"E_rev", "tau_rise", "tau_decay", "g0"};
+  iz.group_param = {"h_min_rel", "h0_rel", "rk5_tol"};
+  iz.scal_param_default = {30.0f, 0.02f, 0.2f, -65.0f, 8.0f, 0.0f, 0.0f,
+                           0.0f, 0.0f};
+  iz.port_param_default = {0.0f, 2.0f, 20.0f, 0.0f};
+  iz.group_param_default = {1.0e-3f, 1.0e-2f, 1.0e-6f};
+  iz.scal_var_default = {-70.0f, 0.2f * -70.0f};  // u = b * V_m
+  iz.port_var_default = {0.0f, 0.0f};
+  iz.fixed_ports = -1;
+
+  m[kUserM1] = b;
+  m[kUserM1].name = "user_m1";
+  m[kUserM2] = b;
+  m[kUserM2].name = "user_m2";
   return m;
 }
 

@@ -147,6 +147,35 @@ struct AeifArgs {
   unsigned long long prop_log_max;
 };
 
+// Other RK5 models (rk5_models.hip): state and parameters in the group's
+// SoA arrays [index][n], reference enum order.
+enum Rk5Kind {
+  kRk5PscAlpha = 0,   // aeif_psc_alpha
+  kRk5PscExp,         // aeif_psc_exp
+  kRk5PscDelta,       // aeif_psc_delta
+  kRk5IzhCondBeta,    // izhikevich_cond_beta
+};
+constexpr int kRk5MaxPort = 8;  // receptor ports of these models
+
+struct Rk5ModelArgs {
+  int kind;                   // Rk5Kind
+  int n, i0, n_port;
+  float *var;                 // [n_var][n]
+  float *param;               // [n_param][n], refractory_step written back
+  double *x;
+  float *h;
+  float h_min;
+  double h_min_acc;           // h_min * (1 + 1e-5)
+  float eps;
+  float dt;
+  double x1;                  // end of this step (neural time, ms)
+  FoldIn fold;
+  SpikeOut out;
+  unsigned long long *cnt;    // [kCounterSlots][2] accepted, rejected
+  int do_step;                // 0: fold only (end-of-simulation flush)
+  const StepClock *clock;
+};
+
 struct IafArgs {
   int n, i0;
   float *I_ex, *I_in, *V, *refr;
@@ -288,6 +317,8 @@ void LaunchPropEval(const unsigned char *pk, const float *feat, int n,
 void LaunchAeifPass2(const AeifArgs &a, int n_port, const DeliveryArgs *d,
                      int d_grid, void *stream);
 void LaunchIaf(const IafArgs &a, void *stream);
+// returns an error message, or nullptr
+const char *LaunchRk5Model(const Rk5ModelArgs &a, void *stream);
 void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);

@@ -405,6 +405,96 @@ static void aeif_rhs(int nvar, const REAL *y, REAL *dy, const void *ctx) {
   }
 }
 
+/* AdEx with current-based synapses, src/aeif_psc_alpha_kernel.h:132-155,
+ * src/aeif_psc_exp_kernel.h:126-146, src/aeif_psc_delta_kernel.h:118-131;
+ * izhikevich_cond_beta, src/izhikevich_cond_beta_kernel.h:145-165 (its
+ * membrane equation in double: the reference's literals are double). */
+typedef struct {
+  int model, n_port;
+  REAL V_th, Delta_T, inv_Delta_T, g_L, E_L, inv_C_m, a, b, inv_tau_w, I_e,
+      V_peak, V_reset, t_ref, c, d;
+  REAL inv_tau_syn[MAX_PORT], w_in[MAX_PORT];
+  REAL E_rev[MAX_PORT], inv_tau_rise[MAX_PORT], inv_tau_decay[MAX_PORT],
+      g0[MAX_PORT];
+  REAL refr;
+} rk5m_par;
+
+static void psc_rhs(int nvar, const REAL *y, REAL *dy, const void *ctx) {
+  const rk5m_par *p = (const rk5m_par *)ctx;
+  (void)nvar;
+  int vp = p->model == ORC_AEIF_PSC_ALPHA ? 2
+           : p->model == ORC_AEIF_PSC_EXP ? 1 : 0;
+  int refr = p->refr > (REAL)0;
+  REAL V = refr ? p->V_reset : ((y[0] < p->V_peak) ? y[0] : p->V_peak);
+  REAL I_syn = 0;
+  if (vp > 0)
+    for (int q = 0; q < p->n_port; q++) I_syn = I_syn + y[2 + vp * q];
+  REAL V_spike = p->Delta_T == (REAL)0
+                     ? (REAL)0
+                     : p->Delta_T * EXPR((V - p->V_th) * p->inv_Delta_T);
+  REAL t = (V - p->E_L) - V_spike;
+  t = (-p->g_L) * t;
+  t = t + I_syn;
+  t = t - y[1];
+  t = t + p->I_e;
+  dy[0] = refr ? (REAL)0 : t * p->inv_C_m;
+  REAL u = p->a * (V - p->E_L);
+  u = u - y[1];
+  dy[1] = u * p->inv_tau_w;
+  for (int q = 0; q < p->n_port; q++) {
+    if (vp == 2) {
+      REAL I = y[2 + 2 * q], I1 = y[3 + 2 * q];
+      dy[3 + 2 * q] = (-I1) * p->inv_tau_syn[q];
+      dy[2 + 2 * q] = FMA(-I, p->inv_tau_syn[q], I1);
+    } else if (vp == 1) {
+      dy[2 + q] = (-y[2 + q]) * p->inv_tau_syn[q];
+    }
+  }
+}
+
+static void izh_rhs(int nvar, const REAL *y, REAL *dy, const void *ctx) {
+  const rk5m_par *p = (const rk5m_par *)ctx;
+  (void)nvar;
+  int refr = p->refr > (REAL)0;
+  REAL V = refr ? p->c : y[0];
+  REAL I_syn = 0;
+  for (int q = 0; q < p->n_port; q++)
+    I_syn = FMA(y[2 + 2 * q], p->E_rev[q] - V, I_syn);
+  double Vd = (double)V;
+  double t = 0.04 * Vd;
+  t = t * Vd;
+  t = fma(5.0, Vd, t);
+  t = t + 140.0;
+  t = t - (double)y[1];
+  t = t + (double)I_syn;
+  t = t + (double)p->I_e;
+  dy[0] = refr ? (REAL)0 : (REAL)t;
+  dy[1] = p->a * FMA(p->b, V, -y[1]);
+  for (int q = 0; q < p->n_port; q++) {
+    REAL g = y[2 + 2 * q], g1 = y[3 + 2 * q];
+    dy[3 + 2 * q] = (-g1) * p->inv_tau_rise[q];
+    dy[2 + 2 * q] = FMA(-g, p->inv_tau_decay[q], g1);
+  }
+}
+
+/* g0 of izhikevich_cond_beta in float, as the reference computes it
+ * (src/izhikevich_cond_beta.cu:57-84) */
+static float orc_izh_g0(float tau_rise, float tau_decay) {
+  float denom1 = tau_decay - tau_rise;
+  float denom2 = 0.0f;
+  if (denom1 != 0.0f) {
+    float t_p = tau_decay * tau_rise * logf(tau_decay / tau_rise) / denom1;
+    denom2 = expf(-t_p / tau_decay) - expf(-t_p / tau_rise);
+  }
+  if (denom2 == 0.0f) return (float)(M_E / tau_decay);
+  return (float)((1.0 / tau_rise - 1.0 / tau_decay) / denom2);
+}
+
+static int is_rk5m(int model) {
+  return model == ORC_AEIF_PSC_ALPHA || model == ORC_AEIF_PSC_EXP ||
+         model == ORC_AEIF_PSC_DELTA || model == ORC_IZH_COND_BETA;
+}
+
 /* ------------------------------------------------------------------ */
 /* Network                                                              */
 /* ------------------------------------------------------------------ */
@@ -531,6 +621,10 @@ static void model_sizes(int model, int n_port, int *nvar, int *nparam) {
   case ORC_IAF_PSC_EXP: *nvar = 4; *nparam = 16; break;
   case ORC_POISSON_GENERATOR: *nvar = 0; *nparam = 4; break;
   case ORC_PARROT_NEURON: *nvar = 3; *nparam = 2; break;
+  case ORC_AEIF_PSC_ALPHA: *nvar = 2 + 2 * n_port; *nparam = 14 + 2 * n_port; break;
+  case ORC_AEIF_PSC_EXP: *nvar = 2 + n_port; *nparam = 14 + n_port; break;
+  case ORC_AEIF_PSC_DELTA: *nvar = 2; *nparam = 14; break;
+  case ORC_IZH_COND_BETA: *nvar = 2 + 2 * n_port; *nparam = 9 + 4 * n_port; break;
   default: *nvar = 0; *nparam = 0; break;
   }
 }
@@ -570,6 +664,31 @@ int orc_net_add_group(orc_net *net, int model, int n, int n_port) {
         } else {
           g->param[(size_t)(14 + pp * q + 1) * n + i] = 2.0f;
         }
+      }
+  } else if (model == ORC_AEIF_PSC_ALPHA || model == ORC_AEIF_PSC_EXP ||
+             model == ORC_AEIF_PSC_DELTA) {
+    /* src/aeif_psc_alpha.cu:24-48 (and the exp / delta variants) */
+    REAL def[14] = {-50.4f, 2.0f, 30.0f, -70.6f, 281.0f, 4.0f, 80.5f, 144.0f,
+                    0.0f, 0.0f, -60.0f, 0.0f, 0.0f, 0.0f};
+    for (int p = 0; p < 14; p++)
+      for (int i = 0; i < n; i++) g->param[(size_t)p * n + i] = def[p];
+    for (int i = 0; i < n; i++) g->var[i] = (REAL)-70.6f;
+    int pp = model == ORC_AEIF_PSC_ALPHA ? 2 : (model == ORC_AEIF_PSC_EXP ? 1 : 0);
+    for (int q = 0; q < n_port && pp > 0; q++)
+      for (int i = 0; i < n; i++) g->param[(size_t)(14 + pp * q) * n + i] = 0.2f;
+  } else if (model == ORC_IZH_COND_BETA) {
+    /* src/izhikevich_cond_beta.cu:24-50 */
+    REAL def[9] = {30.0f, 0.02f, 0.2f, -65.0f, 8.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int p = 0; p < 9; p++)
+      for (int i = 0; i < n; i++) g->param[(size_t)p * n + i] = def[p];
+    for (int i = 0; i < n; i++) {
+      g->var[i] = (REAL)-70.0f;
+      g->var[(size_t)n + i] = (REAL)(0.2f * -70.0f);
+    }
+    for (int q = 0; q < n_port; q++)
+      for (int i = 0; i < n; i++) {
+        g->param[(size_t)(9 + 4 * q + 1) * n + i] = 2.0f;
+        g->param[(size_t)(9 + 4 * q + 2) * n + i] = 20.0f;
       }
   } else if (model == ORC_IAF_PSC_EXP) {
     REAL def[10] = {10.0f, 250.0f, -70.0f, 0.0f, 15.0f, 0.0f, 2.0f, 2.0f,
@@ -807,6 +926,23 @@ int orc_net_calibrate(orc_net *net) {
           }
         }
       }
+    } else if (is_rk5m(g->model)) {
+      int izh = g->model == ORC_IZH_COND_BETA;
+      for (int i = 0; i < g->n; i++) {
+        g->x[i] = net->t0;
+        g->h[i] = g->h0;
+        *P(g, izh ? 7 : 12, i) = 0; /* refractory_step */
+        if (!izh && *P(g, 1, i) <= (REAL)0) *P(g, 9, i) = *P(g, 0, i);
+        for (int q = 0; q < g->n_port; q++) {
+          if (g->model == ORC_AEIF_PSC_ALPHA) {
+            double ts = *P(g, 14 + 2 * q, i);
+            *P(g, 14 + 2 * q + 1, i) = (REAL)(M_E / ts);
+          } else if (izh) {
+            *P(g, 9 + 4 * q + 3, i) = (REAL)orc_izh_g0(
+                (float)*P(g, 9 + 4 * q + 1, i), (float)*P(g, 9 + 4 * q + 2, i));
+          }
+        }
+      }
     } else if (g->model == ORC_IAF_PSC_EXP) {
       for (int i = 0; i < g->n; i++) {
         float pr[6];
@@ -827,6 +963,9 @@ int orc_net_calibrate(orc_net *net) {
     if (g->model == ORC_AEIF_COND_BETA || g->model == ORC_AEIF_COND_ALPHA) den_p = 13;
     if (g->model == ORC_IAF_PSC_EXP) den_p = 9;
     if (g->model == ORC_PARROT_NEURON) den_p = 1;
+    if (g->model == ORC_AEIF_PSC_ALPHA || g->model == ORC_AEIF_PSC_EXP ||
+        g->model == ORC_AEIF_PSC_DELTA) den_p = 13;
+    if (g->model == ORC_IZH_COND_BETA) den_p = 8;
     if (den_p >= 0)
       for (int i = 0; i < g->n; i++)
         net->den_idx[g->i0 + i] = (int)ROUNDR(*P(g, den_p, i) / (REAL)net->dt);
@@ -935,6 +1074,87 @@ static int aeif_update_one(orc_net *net, orc_group *g, int i, double x1,
   return nsp;
 }
 
+/* RK5Update (src/rk5.h:149-162) of the current-based AdEx models and
+ * izhikevich_cond_beta with their ExternalUpdate
+ * (src/aeif_psc_alpha_kernel.h:157-201, src/izhikevich_cond_beta_kernel.h:
+ * 167-206) */
+static int rk5m_update_one(orc_net *net, orc_group *g, int i, double x1,
+                           orc_rk5_count *cnt) {
+  rk5m_par p;
+  memset(&p, 0, sizeof(p));
+  p.model = g->model;
+  p.n_port = g->n_port;
+  int izh = g->model == ORC_IZH_COND_BETA;
+  REAL V_reset, thresh, jump, t_ref;
+  int refr_idx;
+  if (izh) {
+    p.V_th = *P(g, 0, i); p.a = *P(g, 1, i); p.b = *P(g, 2, i);
+    p.c = *P(g, 3, i); p.d = *P(g, 4, i); p.I_e = *P(g, 5, i);
+    p.t_ref = *P(g, 6, i);
+    refr_idx = 7;
+    for (int q = 0; q < g->n_port; q++) {
+      p.E_rev[q] = *P(g, 9 + 4 * q, i);
+      p.inv_tau_rise[q] = (REAL)1 / *P(g, 9 + 4 * q + 1, i);
+      p.inv_tau_decay[q] = (REAL)1 / *P(g, 9 + 4 * q + 2, i);
+    }
+    V_reset = p.c; thresh = p.V_th; jump = p.d; t_ref = p.t_ref;
+  } else {
+    p.V_th = *P(g, 0, i); p.Delta_T = *P(g, 1, i); p.g_L = *P(g, 2, i);
+    p.E_L = *P(g, 3, i);
+    p.inv_Delta_T = (REAL)1 / p.Delta_T;
+    p.inv_C_m = (REAL)1 / *P(g, 4, i);
+    p.a = *P(g, 5, i); p.b = *P(g, 6, i);
+    p.inv_tau_w = (REAL)1 / *P(g, 7, i);
+    p.I_e = *P(g, 8, i); p.V_peak = *P(g, 9, i); p.V_reset = *P(g, 10, i);
+    p.t_ref = *P(g, 11, i);
+    refr_idx = 12;
+    for (int q = 0; q < g->n_port; q++) {
+      if (g->model == ORC_AEIF_PSC_ALPHA)
+        p.inv_tau_syn[q] = (REAL)1 / *P(g, 14 + 2 * q, i);
+      else if (g->model == ORC_AEIF_PSC_EXP)
+        p.inv_tau_syn[q] = (REAL)1 / *P(g, 14 + q, i);
+    }
+    V_reset = p.V_reset; thresh = p.V_peak; jump = p.b; t_ref = p.t_ref;
+  }
+  p.refr = *P(g, refr_idx, i);
+  int nvar = g->nvar;
+  REAL y[64];
+  for (int k = 0; k < nvar; k++) y[k] = *V(g, k, i);
+  double x = g->x[i];
+  REAL h = g->h[i];
+  int nsp = 0;
+  int end = 0;
+  cnt->n_att = 0;
+  while (!end) {
+    REAL hmax = (REAL)(x1 - x);
+    if (orc_rk5_step(nvar, &x, y, &h, g->h_min, hmax, g->epsr,
+                     izh ? izh_rhs : psc_rhs, &p, cnt)) {
+      x = x1;
+    }
+    end = (x >= x1 - (double)g->h_min);
+    if (y[0] < (REAL)-1.0e3) { y[0] = V_reset; y[1] = 0; continue; }
+    if (y[1] < (REAL)-1.0e6 || y[1] > (REAL)1.0e6) {
+      y[0] = V_reset; y[1] = 0; continue;
+    }
+    if (p.refr > (REAL)0) {
+      y[0] = V_reset;
+      if (end) p.refr -= (REAL)1;
+    } else if (y[0] >= thresh) {
+      nsp++;
+      y[0] = V_reset;
+      y[1] = y[1] + jump;
+      REAL r = ROUNDR(t_ref / (REAL)net->dt);
+      int ri = (int)r;
+      p.refr = (REAL)(ri < 0 ? 0 : ri);
+    }
+  }
+  for (int k = 0; k < nvar; k++) *V(g, k, i) = y[k];
+  *P(g, refr_idx, i) = p.refr;
+  g->x[i] = x;
+  g->h[i] = h;
+  return nsp;
+}
+
 /* src/iaf_psc_exp.cu:94-119 */
 static int iaf_update_one(orc_net *net, orc_group *g, int i) {
   REAL *I_ex = V(g, 0, i), *I_in = V(g, 1, i), *Vm = V(g, 2, i),
@@ -1011,6 +1231,25 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
       for (int i = 0; i < g->n; i++) {
         orc_rk5_count cnt = {0, 0, 0};
         ns[i] = aeif_update_one(net, g, i, t, &cnt);
+        tot_acc += cnt.n_acc;
+        tot_rej += cnt.n_rej;
+      }
+      net->cnt_acc += tot_acc;
+      net->cnt_rej += tot_rej;
+      for (int i = 0; i < g->n; i++)
+        for (int s = 0; s < ns[i]; s++)
+          push_spike(net, g->i0 + i, (REAL)1, T, t);
+      free(ns);
+    } else if (is_rk5m(g->model)) {
+      int *ns = (int *)calloc(g->n, sizeof(int));
+      long tot_acc = 0, tot_rej = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(n_threads) schedule(static) \
+    reduction(+ : tot_acc, tot_rej)
+#endif
+      for (int i = 0; i < g->n; i++) {
+        orc_rk5_count cnt = {0, 0, 0};
+        ns[i] = rk5m_update_one(net, g, i, t, &cnt);
         tot_acc += cnt.n_acc;
         tot_rej += cnt.n_rej;
       }
@@ -1112,6 +1351,35 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
           REAL a = acc[(size_t)q * nn + g->i0 + i];
           REAL *g1 = V(g, 3 + 2 * q, i);
           *g1 = FMA(a, *P(g, 14 + pp * q + g0i, i), *g1);
+        }
+    } else if (g->model == ORC_AEIF_PSC_ALPHA) {
+      /* I1_syn += input * I0 */
+      for (int q = 0; q < g->n_port; q++)
+        for (int i = 0; i < g->n; i++) {
+          REAL a = acc[(size_t)q * nn + g->i0 + i];
+          REAL *I1 = V(g, 3 + 2 * q, i);
+          *I1 = FMA(a, *P(g, 14 + 2 * q + 1, i), *I1);
+        }
+    } else if (g->model == ORC_AEIF_PSC_EXP) {
+      /* I_syn += input (weight 1) */
+      for (int q = 0; q < g->n_port; q++)
+        for (int i = 0; i < g->n; i++) {
+          REAL *I = V(g, 2 + q, i);
+          *I = *I + acc[(size_t)q * nn + g->i0 + i];
+        }
+    } else if (g->model == ORC_AEIF_PSC_DELTA) {
+      /* every port into V_m (weight 1, src/aeif_psc_delta.cu:112-120) */
+      for (int q = 0; q < g->n_port; q++)
+        for (int i = 0; i < g->n; i++) {
+          REAL *Vm = V(g, 0, i);
+          *Vm = *Vm + acc[(size_t)q * nn + g->i0 + i];
+        }
+    } else if (g->model == ORC_IZH_COND_BETA) {
+      for (int q = 0; q < g->n_port; q++)
+        for (int i = 0; i < g->n; i++) {
+          REAL a = acc[(size_t)q * nn + g->i0 + i];
+          REAL *g1 = V(g, 3 + 2 * q, i);
+          *g1 = FMA(a, *P(g, 9 + 4 * q + 3, i), *g1);
         }
     } else if (g->model == ORC_IAF_PSC_EXP) {
       for (int q = 0; q < 2; q++)

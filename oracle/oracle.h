@@ -52,7 +52,11 @@ enum {
   ORC_IAF_PSC_EXP = 3,
   ORC_SPIKE_GENERATOR = 4,
   ORC_POISSON_GENERATOR = 5,
-  ORC_PARROT_NEURON = 6
+  ORC_PARROT_NEURON = 6,
+  ORC_AEIF_PSC_ALPHA = 7,
+  ORC_AEIF_PSC_EXP = 8,
+  ORC_AEIF_PSC_DELTA = 9,
+  ORC_IZH_COND_BETA = 10
 };
 
 /* ---- primitives ---- */

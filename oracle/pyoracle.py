@@ -22,6 +22,10 @@ IAF_PSC_EXP = 3
 SPIKE_GENERATOR = 4
 POISSON_GENERATOR = 5
 PARROT_NEURON = 6
+AEIF_PSC_ALPHA = 7
+AEIF_PSC_EXP = 8
+AEIF_PSC_DELTA = 9
+IZH_COND_BETA = 10
 
 MODEL_IDS = {
     "aeif_cond_beta": AEIF_COND_BETA,
@@ -30,6 +34,13 @@ MODEL_IDS = {
     "spike_generator": SPIKE_GENERATOR,
     "poisson_generator": POISSON_GENERATOR,
     "parrot_neuron": PARROT_NEURON,
+    "aeif_psc_alpha": AEIF_PSC_ALPHA,
+    "aeif_psc_exp": AEIF_PSC_EXP,
+    "aeif_psc_delta": AEIF_PSC_DELTA,
+    "izhikevich_cond_beta": IZH_COND_BETA,
+    # src/user_m1_kernel.h, src/user_m2_kernel.h: aeif_cond_beta renamed
+    "user_m1": AEIF_COND_BETA,
+    "user_m2": AEIF_COND_BETA,
 }
 
 AEIF_SCAL_PARAM = ["V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w",
@@ -39,6 +50,8 @@ BETA_PORT_PARAM = ["E_rev", "tau_rise", "tau_decay", "g0"]
 ALPHA_PORT_PARAM = ["E_rev", "tau_syn", "g0"]
 AEIF_SCAL_VAR = ["V_m", "w"]
 AEIF_PORT_VAR = ["g", "g1"]
+IZH_SCAL_PARAM = ["V_th", "a", "b", "c", "d", "I_e", "t_ref",
+                  "refractory_step", "den_delay"]
 IAF_VAR = ["I_syn_ex", "I_syn_in", "V_m_rel", "refractory_step"]
 IAF_PARAM = ["tau_m", "C_m", "E_L", "I_e", "Theta_rel", "V_reset_rel",
              "tau_ex", "tau_in", "t_ref", "den_delay", "P20", "P11ex",
@@ -142,8 +155,17 @@ class Group:
         self.i0, self.n, self.n_port = i0, n, n_port
 
     def var_names(self):
-        if self.model in ("aeif_cond_beta", "aeif_cond_alpha"):
+        if self.model in ("aeif_cond_beta", "aeif_cond_alpha", "user_m1",
+                          "user_m2"):
             return AEIF_SCAL_VAR, AEIF_PORT_VAR
+        if self.model == "aeif_psc_alpha":
+            return AEIF_SCAL_VAR, ["I_syn", "I1_syn"]
+        if self.model == "aeif_psc_exp":
+            return AEIF_SCAL_VAR, ["I_syn"]
+        if self.model == "aeif_psc_delta":
+            return AEIF_SCAL_VAR, []
+        if self.model == "izhikevich_cond_beta":
+            return ["V_m", "u"], ["g", "g1"]
         if self.model == "iaf_psc_exp":
             return IAF_VAR, []
         if self.model == "parrot_neuron":
@@ -151,8 +173,16 @@ class Group:
         return [], []
 
     def param_names(self):
-        if self.model == "aeif_cond_beta":
+        if self.model in ("aeif_cond_beta", "user_m1", "user_m2"):
             return AEIF_SCAL_PARAM, BETA_PORT_PARAM
+        if self.model == "aeif_psc_alpha":
+            return AEIF_SCAL_PARAM, ["tau_syn", "I0"]
+        if self.model == "aeif_psc_exp":
+            return AEIF_SCAL_PARAM, ["tau_syn"]
+        if self.model == "aeif_psc_delta":
+            return AEIF_SCAL_PARAM, []
+        if self.model == "izhikevich_cond_beta":
+            return IZH_SCAL_PARAM, BETA_PORT_PARAM
         if self.model == "aeif_cond_alpha":
             return AEIF_SCAL_PARAM, ALPHA_PORT_PARAM
         if self.model == "iaf_psc_exp":

@@ -219,7 +219,29 @@ def stdp_closed_form(w, dts, common):
 
 def nest_spec(model):
     """Single neuron with 3 ports driven by a spike generator at 10 and
-    400 ms, delays 1/100/130 ms (python/test/test_aeif_cond_beta.py:4-30)."""
+    400 ms, delays 1/100/130 ms (python/test/test_aeif_cond_beta.py:4-30);
+    the current-based AdEx models with 2 ports (1 for aeif_psc_delta),
+    weights 1 / -2, delays 1 / 100 ms (python/test/test_aeif_psc_alpha.py:
+    5-23, test_aeif_psc_exp.py, test_aeif_psc_delta.py)."""
+    if model in ("aeif_psc_alpha", "aeif_psc_exp", "aeif_psc_delta"):
+        params = {"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6,
+                  "g_L": 300.0}
+        n_port = 1
+        if model == "aeif_psc_delta":
+            params["C_m"] = 20000.0
+        else:
+            params["tau_syn"] = [40.0, 20.0]
+            n_port = 2
+        spec = {"groups": [{"model": model, "n": 1, "n_port": n_port,
+                            "params": params},
+                           {"model": "spike_generator", "n": 1}],
+                "spike_gen": [{"node": 1, "times": [10.0, 400.0]}],
+                "conns": [], "records": [(0, "V_m", 0)]}
+        for syn, (w, d) in enumerate(zip([1.0, -2.0], [1.0, 100.0])):
+            spec["conns"].append({"src": [1], "tgt": [0], "weight": [w],
+                                  "delay": [d],
+                                  "port": syn if n_port > 1 else 0})
+        return spec
     if model == "iaf_psc_exp":
         return {"groups": [{"model": "iaf_psc_exp", "n": 1},
                            {"model": "spike_generator", "n": 1}],

@@ -30,7 +30,8 @@ def log(*a):
 
 
 @pytest.mark.parametrize("model", ["aeif_cond_beta", "aeif_cond_alpha",
-                                   "iaf_psc_exp"])
+                                   "iaf_psc_exp", "aeif_psc_alpha",
+                                   "aeif_psc_exp", "aeif_psc_delta"])
 def test_nest_golden_trace(ngpu, model):
     log("case nest_golden", model)
     spec = nb.nest_spec(model)
@@ -439,3 +440,94 @@ def test_graph_steps_stdp_network(ngpu):
     assert len(res[0][0]) > 50
     assert res[1][0] == res[0][0]
     assert res[1][1] == res[0][1]
+
+
+def _rk5_model_net_spec(model, n=300, k=20, seed=7):
+    """Small recurrent network of one of the other RK5 models: constant
+    drive above rheobase (spread per neuron), fixed in-degree k with
+    dyadic weights (every synaptic sum exact in any order) on every port,
+    delays 1-4 ms."""
+    rng = np.random.default_rng(seed)
+    izh = model == "izhikevich_cond_beta"
+    n_port = 1 if model == "aeif_psc_delta" else 2
+    I_e = rng.uniform(4.0, 12.0, n) if izh else rng.uniform(500.0, 900.0, n)
+    params = {"I_e": I_e.astype(np.float32)}
+    if model in ("aeif_psc_alpha", "aeif_psc_exp"):
+        params["tau_syn"] = [2.0, 5.0]
+    if izh:
+        params["E_rev"] = [0.0, -85.0]
+    src = rng.integers(0, n, n * k)
+    tgt = np.repeat(np.arange(n), k)
+    port = rng.integers(0, n_port, n * k)
+    if izh:
+        w = rng.choice([0.0625, 0.125, 0.25], n * k)
+    elif model == "aeif_psc_delta":
+        w = rng.choice([-0.5, 0.25, 0.5], n * k)
+    else:
+        w = rng.choice([-8.0, 4.0, 16.0], n * k)
+    d = rng.integers(10, 41, n * k) * 0.1
+    conns = [{"src": src[port == q].tolist(), "tgt": tgt[port == q].tolist(),
+              "weight": w[port == q].tolist(), "delay": d[port == q].tolist(),
+              "port": q} for q in range(n_port)]
+    return {"groups": [{"model": model, "n": n, "n_port": n_port,
+                        "params": params, "rk5": {"rk5_tol": 1e-4}}],
+            "conns": conns, "records": []}
+
+
+@pytest.mark.parametrize("model", ["aeif_psc_alpha", "aeif_psc_exp",
+                                   "aeif_psc_delta", "izhikevich_cond_beta"])
+def test_rk5_model_network_bit_exact(ngpu, model):
+    """Recurrent 300-neuron network of each of the other RK5 models,
+    200 ms: spike trains equal the FP32 oracle's spike for spike."""
+    log("case rk5_model_network", model)
+    spec = _rk5_model_net_spec(model)
+    eng, orc, net = _run_both(ngpu, spec, 2000, 300)
+    log("spikes engine", len(eng), "oracle", len(orc), net.counters())
+    assert len(orc) > 100
+    assert eng == orc
+
+
+def test_izhikevich_trace_bit_exact(ngpu):
+    """Single izhikevich_cond_beta neuron under constant current plus
+    conductance input on two ports: V_m, u and g traces equal the FP32
+    oracle's value for value over 300 ms."""
+    log("case izhikevich_trace")
+    spec = {"groups": [{"model": "izhikevich_cond_beta", "n": 1,
+                        "n_port": 2,
+                        "params": {"I_e": 5.0, "E_rev": [0.0, -85.0],
+                                   "tau_rise": [1.0, 2.0],
+                                   "tau_decay": [5.0, 20.0]}},
+                       {"model": "spike_generator", "n": 1}],
+            "spike_gen": [{"node": 1, "times": [20.0, 100.0, 180.0]}],
+            "conns": [{"src": [1], "tgt": [0], "weight": [0.5],
+                       "delay": [1.0], "port": 0},
+                      {"src": [1], "tgt": [0], "weight": [0.3],
+                       "delay": [30.0], "port": 1}],
+            "records": [(0, "V_m", 0), (0, "u", 0), (0, "g", 0),
+                        (0, "g", 1)]}
+    _, recs = nb.build_engine(ngpu, spec)
+    ngpu.Simulate(300.0)
+    cols = [ngpu.GetRecordDataArray(r) for r in recs]
+    data = np.column_stack([cols[0][:, 0]] + [c[:, 1] for c in cols])
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(3000)
+    orc = net.records()
+    assert orc.shape == data.shape
+    assert np.sum(data[:, 1] >= 29.0) == 0     # reset below threshold
+    assert np.array_equal(orc[:, 1:].astype(np.float32), data[:, 1:])
+
+
+@pytest.mark.parametrize("model", ["user_m1", "user_m2"])
+def test_user_models_match_aeif_cond_beta(ngpu, model):
+    """user_m1 / user_m2 (aeif_cond_beta under other names in the
+    reference) give the aeif_cond_beta trace value for value."""
+    log("case user_model", model)
+    spec = nb.nest_spec("aeif_cond_beta")
+    _, recs = nb.build_engine(ngpu, spec)
+    ngpu.Simulate(500.0)
+    ref = ngpu.GetRecordDataArray(recs[0])
+    spec["groups"][0]["model"] = model
+    _, recs = nb.build_engine(ngpu, spec)
+    ngpu.Simulate(500.0)
+    got = ngpu.GetRecordDataArray(recs[0])
+    assert np.array_equal(got, ref)

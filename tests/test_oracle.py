@@ -250,7 +250,8 @@ def test_rk5_adaptive_tolerance(eps):
 # neuron models against the reference's golden NEST traces
 # ---------------------------------------------------------------------
 @pytest.mark.parametrize("model", ["aeif_cond_beta", "aeif_cond_alpha",
-                                   "iaf_psc_exp"])
+                                   "iaf_psc_exp", "aeif_psc_alpha",
+                                   "aeif_psc_exp", "aeif_psc_delta"])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_nest_golden_trace_oracle(model, prec):
     """python/test/test_<model>.py: NEST row i <-> record row 10 i + 20,
@@ -376,3 +377,117 @@ def test_c2_spec_shape():
         assert (w > 0).all() if pop % 2 == 0 else (w < 0).all()
         assert np.all(np.asarray(c["delay"]) >= 0.1)
         assert np.all(w == np.round(w))
+
+
+# ---------------------------------------------------------------------
+# izhikevich_cond_beta and user_m1 / user_m2 (no golden vectors in the
+# reference): closed forms, an independent integration, identities
+# ---------------------------------------------------------------------
+def test_izhikevich_default_state_is_fixed_point():
+    """With I_e = 0 and no input, 0.04 V^2 + 5 V + 140 - u = 0 and
+    b V - u = 0 hold at the defaults V = -70, u = b V = -14 (the stable
+    root of 0.04 V^2 + 4.8 V + 140 = 0): the state stays there."""
+    spec = {"groups": [{"model": "izhikevich_cond_beta", "n": 1,
+                        "n_port": 1}],
+            "records": [(0, "V_m", 0), (0, "u", 0)]}
+    for prec in ("f64", "f32"):
+        net, _ = nb.build_oracle(spec, prec)
+        net.run(2000)
+        r = net.records()
+        assert np.allclose(r[:, 1], -70.0, atol=1e-5)
+        assert np.allclose(r[:, 2], -14.0, atol=1e-5)
+
+
+def _izh_scipy_spikes(I_e, t_end, a=0.02, b=0.2, c=-65.0, d=8.0,
+                      V_th=30.0):
+    """Spike times of the Izhikevich equations under constant current from
+    scipy's LSODA with threshold events (independent of the oracle)."""
+    from scipy.integrate import solve_ivp
+
+    def f(t, y):
+        V, u = y
+        return [0.04 * V * V + 5.0 * V + 140.0 - u + I_e, a * (b * V - u)]
+
+    def ev(t, y):
+        return y[0] - V_th
+    ev.terminal = True
+    ev.direction = 1
+    t, y = 0.0, [-70.0, b * -70.0]
+    spikes = []
+    while t < t_end:
+        sol = solve_ivp(f, (t, t_end), y, method="LSODA", events=ev,
+                        rtol=1e-10, atol=1e-10, max_step=0.05)
+        if sol.status == 1:
+            t = float(sol.t_events[0][0])
+            spikes.append(t)
+            y = [c, float(sol.y_events[0][0][1]) + d]
+        else:
+            break
+    return np.array(spikes)
+
+
+def test_izhikevich_spike_times_match_independent_integration():
+    """Regular spiking Izhikevich neuron under I_e = 10 over 500 ms: the
+    FP64 oracle (RK5, tolerance 1e-8) against an independent LSODA
+    integration with exact threshold events.  The oracle resets at the end
+    of the substep that crosses V_th (ExternalUpdate after each substep,
+    src/rk5.h:149-162) and stamps spikes at the end of the time step, so
+    the first spike agrees to within one step (0.1 ms) and every
+    inter-spike interval to within one step plus that substep (0.12 ms);
+    the spike counts are equal."""
+    spec = {"groups": [{"model": "izhikevich_cond_beta", "n": 1,
+                        "n_port": 1, "params": {"I_e": 10.0},
+                        "rk5": {"rk5_tol": 1e-8}}],
+            "records": []}
+    net, _ = nb.build_oracle(spec, "f64")
+    net.run(5000)
+    st, nd, ct = net.spikes()
+    t_orc = (np.asarray(st, dtype=np.float64) + 1) * 0.1
+    t_ref = _izh_scipy_spikes(10.0, 500.0)
+    assert len(t_ref) >= 5
+    assert len(t_orc) == len(t_ref)
+    assert 0.0 <= t_orc[0] - t_ref[0] <= 0.1 + 1e-9
+    assert np.all(np.abs(np.diff(t_orc) - np.diff(t_ref)) <= 0.12)
+
+
+def test_user_models_are_aeif_cond_beta():
+    """user_m1 and user_m2 are aeif_cond_beta under other names in the
+    reference (src/user_m1_kernel.h, src/user_m2_kernel.h): identical
+    traces on the NEST test setup."""
+    base = nb.nest_spec("aeif_cond_beta")
+    net, _ = nb.build_oracle(base, "f32")
+    net.run(3000)
+    ref = net.records()
+    for m in ("user_m1", "user_m2"):
+        spec = nb.nest_spec("aeif_cond_beta")
+        spec["groups"][0]["model"] = m
+        net, _ = nb.build_oracle(spec, "f32")
+        net.run(3000)
+        assert np.array_equal(net.records(), ref)
+
+
+def test_izhikevich_conductance_peak_is_weight():
+    """izhikevich_cond_beta's g0 normalises the beta conductance to peak
+    at the synaptic weight, t_p = td tr ln(td/tr) / (td - tr) after the
+    spike arrives (src/izhikevich_cond_beta.cu:57-84): a single input of
+    weight 0.5 nS on port 1 (tau_rise 2, tau_decay 20 ms) peaks at 0.5 nS,
+    5.117 ms after its arrival."""
+    spec = {"groups": [{"model": "izhikevich_cond_beta", "n": 1,
+                        "n_port": 2,
+                        "params": {"tau_rise": [1.0, 2.0],
+                                   "tau_decay": [10.0, 20.0]}},
+                       {"model": "spike_generator", "n": 1}],
+            "spike_gen": [{"node": 1, "times": [10.0]}],
+            "conns": [{"src": [1], "tgt": [0], "weight": [0.5],
+                       "delay": [1.0], "port": 1}],
+            "records": [(0, "g", 1)]}
+    for prec in ("f64", "f32"):
+        net, _ = nb.build_oracle(spec, prec)
+        net.run(600)
+        r = net.records()
+        k = int(np.argmax(r[:, 1]))
+        t_p = 20.0 * 2.0 * math.log(10.0) / 18.0
+        # the input is folded at the end of step 110 (spike at 10 ms
+        # delivered after 1 ms) and integrated from t = 11.1 ms
+        assert abs(r[k, 1] - 0.5) <= 2e-4
+        assert abs(r[k, 0] - (11.1 + t_p)) <= 0.1 + 1e-6

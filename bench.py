@@ -466,6 +466,9 @@ def main():
     ap.add_argument("--no-readme", action="store_true")
     ap.add_argument("--only-readme", action="store_true",
                     help="measure only the README network (profiling)")
+    ap.add_argument("--only", choices=["c1", "readme", "c2", "c3", "c4"],
+                    help="measure one workload and print its object only "
+                         "(profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workloads", default="c2,c3,c4",
                     help="further configurations to measure after c1 "
@@ -488,10 +491,29 @@ def main():
     from neurongpu_amd import neurongpu as ngpu
 
     if args.only_readme:
-        neuron, n_syn = build_readme(ngpu, args.neurons)
-        rd = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
-                     args.timing_steps)
-        print(json.dumps({"readme_network": rd}), flush=True)
+        args.only = "readme"
+    if args.only:
+        from neurongpu_amd import workloads as wl
+        after = None
+        if args.only == "c1":
+            neuron, n_syn = build_c1(ngpu, args.neurons)
+        elif args.only == "readme":
+            neuron, n_syn = build_readme(ngpu, args.neurons)
+        elif args.only == "c2":
+            neuron, n_syn = build_c2(ngpu, args.c2_scale)
+        elif args.only == "c3":
+            neuron, n_syn = build_c3(ngpu, args.c3_scale)
+        else:
+            from neurongpu_amd.autotune import autotune
+            inst = wl.c4_instances()[:args.c4_instances]
+            neuron, _, n_syn = wl.build_ensemble(ngpu, inst)
+            after = (lambda: autotune(ngpu, steps=40, log=log))
+        r = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
+                    args.timing_steps, after_warmup=after)
+        r["roofline"] = roofline(r)
+        out = {args.only: r}
+        copy_calibration(args, out)
+        print(json.dumps(out), flush=True)
         return
     log("building c1 (%d neurons)" % args.neurons)
     neuron, n_syn = build_c1(ngpu, args.neurons)
@@ -644,16 +666,23 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_neurons, args.cpu_steps)
-    if args.copy_calib_mib > 0:
-        import torch
-        n = args.copy_calib_mib * (1 << 20) // 4
-        x = torch.ones(n, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = 4 * n
+    copy_calibration(args, out)
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(args, out):
+    """A plain device copy of --copy-calib-mib MiB (3 launches), whose PMC
+    counters scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    if args.copy_calib_mib <= 0:
+        return
+    import torch
+    n = args.copy_calib_mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    out["copy_calibration_bytes"] = 4 * n
 
 
 if __name__ == "__main__":

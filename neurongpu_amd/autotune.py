@@ -19,7 +19,7 @@ import time
 
 CANDIDATES = {
     "rk5_budget": [1, 2],
-    "rk5_pass2_grid": [256, 1024, 4096],
+    "rk5_pass2_grid": [256, 64, 1024],
     "delivery_mode": [2, 0, 1],
     "delivery_grid": [1024, 2048, 4096],
     "delivery_tile_max": [8, 16, 64],

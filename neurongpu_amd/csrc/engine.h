@@ -167,7 +167,7 @@ class Engine {
   // two-pass RK5 update (false: one neuron per lane until done)
   bool rk5_two_pass = true;
   int rk5_budget = 1;          // RK5 attempts per neuron in the first pass
-  int rk5_pass2_grid = 1024;   // blocks of the second pass
+  int rk5_pass2_grid = 256;   // blocks of the second pass
   bool calibrated = false;
   bool first_simulation = true;
   bool in_simulation = false;

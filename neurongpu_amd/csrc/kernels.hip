@@ -578,7 +578,9 @@ __global__ __launch_bounds__(kBlock) void aeif_pass1_kernel(AeifArgs A) {
 // entry as soon as the current neuron is done, so lanes stay busy while
 // neurons need different numbers of attempts, and waves beyond the list
 // end exit at once.
-template <int P, bool UNI>
+// PROP: the learned proposer's inference is compiled in (its weights in
+// LDS, the MFMA registers); without it the pass keeps its occupancy.
+template <int P, bool UNI, bool PROP>
 __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
                                                 uint32_t vblock,
                                                 uint32_t vgrid) {
@@ -589,8 +591,9 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       A.strag_n2 + kStragLists * (aeif_gs(A) & 1);
   __shared__ uint32_t s_pre[kStragLists + 1];
   // proposer weights, staged once per block (read at every step start)
-  __shared__ __attribute__((aligned(16))) unsigned char s_pk[kPropPkBytes];
-  if (A.prop_pk) {
+  __shared__ __attribute__((aligned(16)))
+  unsigned char s_pk[PROP ? kPropPkBytes : 16];
+  if (PROP && A.prop_pk) {
     const uint4 *src = reinterpret_cast<const uint4 *>(A.prop_pk);
     uint4 *dst = reinterpret_cast<uint4 *>(s_pk);
     for (int k = threadIdx.x; k < kPropPkBytes / 16; k += kBlock)
@@ -660,7 +663,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
 #pragma unroll
     for (int k = 0; k < kPropIn; k++) feat[k] = 0.0f;
     float err = 1.0f, hh = 0.0f;
-    if (A.prop_pk) {
+    if (PROP && A.prop_pk) {
       // learned step size for the lanes that start an RK5 step; the MLP
       // runs on the whole wave
       bool want = have && need_k1;
@@ -720,9 +723,9 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
   aeif_count<P>(A, n_acc, n_rej);
 }
 
-template <int P, bool UNI>
+template <int P, bool UNI, bool PROP>
 __global__ __launch_bounds__(kBlock) void aeif_pass2_kernel(AeifArgs A) {
-  aeif_pass2_body<P, UNI>(A, blockIdx.x, gridDim.x);
+  aeif_pass2_body<P, UNI, PROP>(A, blockIdx.x, gridDim.x);
 }
 
 template <bool PLASTIC>
@@ -769,13 +772,13 @@ __device__ __forceinline__ RevArgs clocked(const RevArgs &R) {
 // spikes of earlier steps and writes only the input accumulators), so the
 // latency-bound second pass and the atomic-bound delivery share the chip
 // without a second stream.
-template <int P, bool UNI>
+template <int P, bool UNI, bool PROP>
 __global__ __launch_bounds__(kBlock) void aeif_tail_kernel(AeifArgs A,
                                                            DeliveryArgs D,
                                                            uint32_t n2) {
   clock_step(D.clock, D.clock_next);
   if (blockIdx.x < n2)
-    aeif_pass2_body<P, UNI>(A, blockIdx.x, n2);
+    aeif_pass2_body<P, UNI, PROP>(A, blockIdx.x, n2);
   else
     delivery_body<false>(clocked(D), blockIdx.x - n2, gridDim.x - n2);
 }
@@ -913,6 +916,27 @@ __device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
                                                       uint32_t *s_part) {
   const int D = A.D;
   const int tid = threadIdx.x;
+  if (D <= 64) {
+    // one wave: scan of the D counts through cross-lane shuffles
+    if (tid < 64) {
+      uint32_t n = 0;
+      if (tid < D) {
+        long long s = A.gs - 1 - tid;
+        if (s >= 0) n = A.ring_n[s % A.R];
+      }
+      uint32_t incl = n;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = __shfl_up(incl, off);
+        if (tid >= off) incl += v;
+      }
+      if (tid < D) s_pre[tid] = incl - n;
+      if (tid == 63) s_pre[D] = incl;
+    }
+    __syncthreads();
+    if (vblock == 0 && tid == 0) A.ring_n[(A.gs + 1) % A.R] = 0u;
+    return s_pre[D];
+  }
   // per-thread partial sums of the due spike counts over d
   const int per = (D + kBlock - 1) / kBlock;
   uint32_t run = 0;
@@ -1483,23 +1507,35 @@ void launch_aeif_p(const AeifArgs &a, hipStream_t s) {
   }
 }
 
+template <int P, bool UNI, bool PROP>
+void launch_pass2_v(const AeifArgs &a, const DeliveryArgs *d, int grid2,
+                    int d_grid, hipStream_t s) {
+  if (d)
+    hipLaunchKernelGGL((aeif_tail_kernel<P, UNI, PROP>), dim3(grid2 + d_grid),
+                       dim3(kBlock), delivery_lds_bytes(d->D, 0), s, a, *d,
+                       (uint32_t)grid2);
+  else
+    hipLaunchKernelGGL((aeif_pass2_kernel<P, UNI, PROP>), dim3(grid2),
+                       dim3(kBlock), 0, s, a);
+}
+
 template <int P>
 void launch_aeif_pass2_p(const AeifArgs &a, const DeliveryArgs *d,
                          int d_grid, hipStream_t s) {
   int grid2 = a.pass2_grid > 0 ? a.pass2_grid : 256;
-  const size_t lds = d ? delivery_lds_bytes(d->D, 0) : 0;
-  if (d && a.uniform)
-    hipLaunchKernelGGL((aeif_tail_kernel<P, true>), dim3(grid2 + d_grid),
-                       dim3(kBlock), lds, s, a, *d, (uint32_t)grid2);
-  else if (d)
-    hipLaunchKernelGGL((aeif_tail_kernel<P, false>), dim3(grid2 + d_grid),
-                       dim3(kBlock), lds, s, a, *d, (uint32_t)grid2);
-  else if (a.uniform)
-    hipLaunchKernelGGL((aeif_pass2_kernel<P, true>), dim3(grid2), dim3(kBlock),
-                       0, s, a);
-  else
-    hipLaunchKernelGGL((aeif_pass2_kernel<P, false>), dim3(grid2), dim3(kBlock),
-                       0, s, a);
+  // the proposer's inference exists for groups of up to kPropMaxPort ports
+  constexpr bool kProp = P <= kPropMaxPort;
+  if (a.prop_pk && kProp) {
+    if (a.uniform)
+      launch_pass2_v<P, true, kProp>(a, d, grid2, d_grid, s);
+    else
+      launch_pass2_v<P, false, kProp>(a, d, grid2, d_grid, s);
+  } else {
+    if (a.uniform)
+      launch_pass2_v<P, true, false>(a, d, grid2, d_grid, s);
+    else
+      launch_pass2_v<P, false, false>(a, d, grid2, d_grid, s);
+  }
 }
 
 }  // namespace

@@ -25,6 +25,9 @@ constexpr int kPropNParam = kPropB3 + 1;                // 1313
 constexpr int kPropSample = 8;   // floats per logged sample: 6 features,
                                  // target, spare
 constexpr float kPropSafety = 0.9f;
+// the second RK5 pass carries the proposer for groups of up to this many
+// receptor ports
+constexpr int kPropMaxPort = 4;
 // trust region of a proposal relative to the plain controller's step
 constexpr float kPropShrinkMax = 0.5f;
 constexpr float kPropGrowMax = 32.0f;

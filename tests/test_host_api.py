@@ -335,7 +335,7 @@ def test_kernel_status(ngpu):
 def test_launch_settings(ngpu):
     """Launch-shape settings of this implementation: defaults, round trip
     and range checks (reported like the reference's parameter errors)."""
-    defaults = {"rk5_two_pass": 1, "rk5_budget": 1, "rk5_pass2_grid": 1024,
+    defaults = {"rk5_two_pass": 1, "rk5_budget": 1, "rk5_pass2_grid": 256,
                 "fuse_delivery": 1, "delivery_grid": 2048,
                 "delivery_tile_max": 64, "delivery_mode": 2,
                 "delivery_bin_capacity": 1 << 27,

@@ -233,6 +233,12 @@ class Engine {
   int delivery_mode = 2;
   int delivery_bin_capacity = 1 << 27;  // events in the binned-mode buffer
   int delivery_auto_events = 500000;
+  // 2: binned delivery on a second stream, beside the second RK5 pass
+  // (they are independent without plastic synapses); 1: one stream
+  int delivery_streams = 2;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void EnsureStream2();
   unsigned long long *h_ev_snap = nullptr;  // pinned copy of the counters
   hipEvent_t ev_snap = nullptr;
   bool snap_pending = false, auto_binned = false;

@@ -372,6 +372,7 @@ def test_ensemble_routing_matches_instances_alone(ngpu):
     {"delivery_grid": 256, "delivery_tile_max": 8, "fuse_delivery": 0},
     {"delivery_grid": 4096, "delivery_tile_max": 1},
     {"delivery_mode": 1},
+    {"delivery_mode": 1, "delivery_streams": 1},
     {"delivery_mode": 1, "delivery_grid": 64, "delivery_tile_max": 4},
 ])
 def test_launch_shapes_do_not_change_results(ngpu, settings):

@@ -339,7 +339,8 @@ def test_launch_settings(ngpu):
                 "fuse_delivery": 1, "delivery_grid": 2048,
                 "delivery_tile_max": 64, "delivery_mode": 2,
                 "delivery_bin_capacity": 1 << 27,
-                "delivery_auto_events": 500000, "graph_steps": 0}
+                "delivery_auto_events": 500000, "graph_steps": 0,
+                "delivery_streams": 2}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)
@@ -351,6 +352,6 @@ def test_launch_settings(ngpu):
     for k, bad in (("delivery_mode", 3), ("delivery_tile_max", 3),
                    ("delivery_grid", 0), ("rk5_budget", 0),
                    ("delivery_bin_capacity", 0), ("graph_steps", -1),
-                   ("graph_steps", 7)):
+                   ("graph_steps", 7), ("delivery_streams", 3)):
         with pytest.raises(ValueError):
             ngpu.SetKernelStatus(k, bad)

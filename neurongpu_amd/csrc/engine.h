@@ -241,10 +241,20 @@ class Engine {
   void EnsureStream2();
   unsigned long long *h_ev_snap = nullptr;  // pinned copy of the counters
   hipEvent_t ev_snap = nullptr;
-  bool snap_pending = false, auto_binned = false;
+  bool snap_pending = false;
   long long snap_gs = -1, prev_snap_gs = -1;
   unsigned long long prev_snap_total = 0;
-  bool AutoBinned();
+  double recent_events = -1.0;  // events per step of the recent past (<0:
+                                // not measured yet)
+  void UpdateEventRate();
+  // grid of the atomic delivery: a small one while recent steps carried
+  // almost no events (results do not depend on the grid)
+  int DeliveryGridNow() const {
+    return (recent_events >= 0.0 && recent_events < 4096.0 &&
+            delivery_grid > 256)
+               ? 256
+               : delivery_grid;
+  }
   // hipGraph replay of whole steps: graph_steps steps per captured graph
   // (0: never, the default: on MI355X with ROCm 7.2 a replayed step of
   // the 1M-neuron network took longer than the same launches issued one by
@@ -255,7 +265,7 @@ class Engine {
     hipGraphExec_t exec = nullptr;
     int steps = 0;
   };
-  StepGraph graphs[2];
+  StepGraph graphs[3];  // atomic, binned, atomic with the quiet grid
   DBuf<StepClock> d_clock;
   bool clock_mode = false;    // launches being captured take d_clock
   int clock_slot = 0;         // clock of the step being captured

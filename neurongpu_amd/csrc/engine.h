@@ -34,6 +34,7 @@ enum ModelId {
   kIzhikevichCondBeta,
   kUserM1,          // src/user_m1_kernel.h: aeif_cond_beta under another name
   kUserM2,          // src/user_m2_kernel.h: the same
+  kPoissonLegacy,   // CreatePoissonGenerator (src/poisson.cu); no model name
   kNumModels
 };
 
@@ -67,6 +68,9 @@ struct ModelDesc {
 
 const ModelDesc &Model(ModelId id);
 int FindModel(const std::string &name);
+// inversion thresholds of Poisson(lambda) (engine.cpp); returns the table
+// length, or -1 when max_len entries do not reach the top of the CDF
+int PoissonTable(double lambda, uint32_t *tab, int max_len);
 
 // device buffer owned by the engine
 template <class T>
@@ -306,6 +310,12 @@ class Engine {
   DBuf<float> d_dir_w, d_dir_delay, d_gen_origin, d_gen_start, d_gen_stop;
   DBuf<uint8_t> d_dir_port;
   bool pois_dirty = true;
+  // the group of CreatePoissonGenerator (src/poisson.cu:98-147): mean count
+  // per step, its CDF table, and the count above which a draw is dropped
+  int legacy_group = -1;
+  float legacy_lambda = 0.0f;
+  std::vector<uint32_t> legacy_tab;
+  DBuf<uint32_t> d_legacy_tab;
   // statistics
   DBuf<unsigned long long> d_cnt_rk5, d_cnt_events;
   unsigned long long n_spikes_total = 0;

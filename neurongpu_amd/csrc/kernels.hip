@@ -902,6 +902,42 @@ __global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
 }
 
 // ---------------------------------------------------------------------
+// CreatePoissonGenerator nodes (src/poisson.cu:27-54): Philox word at
+// counter (node, T, kLegacyPoissonTag), inverted through the CDF table.
+// The reference pushes these spikes before it advances the step's time
+// (src/neurongpu.cu:461-470), so they carry the previous step's time index
+// and neural time; they enter the same ring slot as this step's spikes.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void legacy_pois_kernel(LegacyPoisArgs A) {
+  StepNow sn = step_now(A.out, nullptr, A.clock);
+  const long long it = A.clock ? A.clock->it : A.it;
+  const double t0 = A.clock ? A.clock->t0 : A.t0;
+  const double dt = A.clock ? A.clock->dt : A.dt;
+  const long long T = sn.T;
+  sn.T = T - 1;
+  sn.t = t0 + dt * (double)it;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  bool active = i < A.n;
+  int ii = active ? i : 0;
+  int node = A.i0 + ii;
+  int nsp = 0;
+  float height = 0.0f;
+  if (active) {
+    uint32_t u = dev_philox_w0((uint32_t)node, (uint32_t)T,
+                               (uint32_t)((uint64_t)T >> 32),
+                               kLegacyPoissonTag, A.key0, A.key1);
+    int k = 0;
+    while (u > A.tab[k]) k++;   // the table ends with 0xFFFFFFFF
+    if ((float)k > A.max_count) k = 0;
+    if (k > 0) {
+      nsp = 1;
+      height = (float)k;
+    }
+  }
+  emit_spikes(A.out, sn, active, ii, node, nsp, height);
+}
+
+// ---------------------------------------------------------------------
 // Delivery.  Pairs (spike, delay slot d) due at global step gs are the
 // spikes of ring slot gs-1-d combined with the source's synapse segment of
 // delay index d.  Wavefronts take tiles of 64 pairs; a wavefront scans the
@@ -1594,6 +1630,13 @@ void LaunchRelay(const RelayArgs &a, void *stream) {
   int grid = (a.n + kBlock - 1) / kBlock;
   if (grid > 0)
     hipLaunchKernelGGL(relay_kernel, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+}
+
+void LaunchLegacyPoisson(const LegacyPoisArgs &a, void *stream) {
+  int grid = (a.n + kBlock - 1) / kBlock;
+  if (grid > 0)
+    hipLaunchKernelGGL(legacy_pois_kernel, dim3(grid), dim3(kBlock), 0,
                        (hipStream_t)stream, a);
 }
 

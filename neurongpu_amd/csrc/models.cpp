@@ -132,6 +132,12 @@ std::vector<ModelDesc> MakeModels() {
   m[kUserM1].name = "user_m1";
   m[kUserM2] = b;
   m[kUserM2].name = "user_m2";
+
+  // CreatePoissonGenerator nodes (src/neurongpu.cu:245-265): a BaseNeuron
+  // group with no variables, parameters or receptor ports
+  ModelDesc &lp = m[kPoissonLegacy];
+  lp.name = "base_neuron";
+  lp.fixed_ports = 0;
   return m;
 }
 
@@ -144,7 +150,7 @@ const ModelDesc &Model(ModelId id) {
 
 int FindModel(const std::string &name) {
   for (int i = 0; i < kNumModels; i++)
-    if (name == Model((ModelId)i).name) return i;
+    if (i != kPoissonLegacy && name == Model((ModelId)i).name) return i;
   return -1;
 }
 

@@ -25,6 +25,7 @@ constexpr long long kLastSpikeTimeGuard = 0x70000000LL;
 constexpr int kMaxAttemptsPerStep = 100000;
 constexpr int kCounterSlots = 1024;   // spread statistic counters over L2
 constexpr uint32_t kPoissonTag = 0x504F4953u;
+constexpr uint32_t kLegacyPoissonTag = 0x4C504F49u;
 constexpr int kStragRecBytes = 24;    // resume record of the two-pass update
 
 // Step clock in device memory.  Kernels launched with a non-null clock
@@ -212,6 +213,20 @@ struct SpikeGenArgs {
   const StepClock *clock;
 };
 
+// nodes of CreatePoissonGenerator (src/poisson.cu:27-54): one Poisson
+// count per node and step, dropped when above 5 lambda, sent as the height
+// of a spike
+struct LegacyPoisArgs {
+  int n, i0;
+  const uint32_t *tab;          // CDF thresholds of Poisson(lambda)
+  float max_count;              // 5 lambda
+  uint32_t key0, key1;          // Philox key (the random seed)
+  long long it;                 // step index within this Simulate call
+  double t0, dt;                // neural time at the call's start, time step
+  SpikeOut out;
+  const StepClock *clock;
+};
+
 struct DeliveryArgs {
   const uint32_t *ring_src;     // [R][cap]
   const float *ring_h;
@@ -321,6 +336,7 @@ void LaunchIaf(const IafArgs &a, void *stream);
 const char *LaunchRk5Model(const Rk5ModelArgs &a, void *stream);
 void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
+void LaunchLegacyPoisson(const LegacyPoisArgs &a, void *stream);
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
 void LaunchClockSet(StepClock *c, const StepClock &v, void *stream);
 // Binned delivery: count, scan, scatter, reduce, sum (five launches).

@@ -31,7 +31,10 @@ def build_from_spec(ngpu, spec):
     ngpu.SetRandomSeed(spec.get("seed", 54321))
     groups = []
     for g in spec["groups"]:
-        ns = ngpu.Create(g["model"], g["n"], g.get("n_port", 1))
+        if g["model"] == "legacy_poisson_generator":
+            ns = ngpu.CreatePoissonGenerator(g["n"], g["rate"])
+        else:
+            ns = ngpu.Create(g["model"], g["n"], g.get("n_port", 1))
         groups.append(ns)
         for k, v in g.get("params", {}).items():
             _set(ngpu, ns, k, v)

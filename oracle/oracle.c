@@ -512,6 +512,8 @@ typedef struct {
   int **sg_idx;
   /* poisson generator */
   float *pg_rate, *pg_origin, *pg_start, *pg_stop;
+  float lp_lam;      /* CreatePoissonGenerator: mean count per step */
+  uint32_t *lp_tab;  /* and its CDF table */
 } orc_group;
 
 typedef struct {
@@ -599,6 +601,7 @@ static void grp_free(orc_group *g) {
     free(g->sg_idx);
   }
   free(g->pg_rate); free(g->pg_origin); free(g->pg_start); free(g->pg_stop);
+  free(g->lp_tab);
 }
 
 void orc_net_free(orc_net *net) {
@@ -635,7 +638,9 @@ int orc_net_add_group(orc_net *net, int model, int n, int n_port) {
   g->model = model;
   g->n = n;
   if (model == ORC_IAF_PSC_EXP) n_port = 2;
-  if (model == ORC_SPIKE_GENERATOR || model == ORC_POISSON_GENERATOR) n_port = 0;
+  if (model == ORC_SPIKE_GENERATOR || model == ORC_POISSON_GENERATOR ||
+      model == ORC_POISSON_LEGACY)
+    n_port = 0;
   if (model == ORC_PARROT_NEURON) n_port = 2;
   g->n_port = n_port;
   g->i0 = net->n_node;
@@ -714,6 +719,20 @@ int orc_net_add_group(orc_net *net, int model, int n, int n_port) {
   net->n_node += n;
   if (n_port > net->max_port) net->max_port = n_port;
   return net->groups[net->n_group++].i0;
+}
+
+int orc_net_legacy_rate(orc_net *net, int group, float rate) {
+  orc_group *g = &net->groups[group];
+  /* float product, then a double division stored as float
+   * (src/neurongpu.cu:261) */
+  float dt = (float)net->dt_d;
+  g->lp_lam = (float)((double)(rate * dt) / 1000.0);
+  free(g->lp_tab);
+  g->lp_tab = (uint32_t *)malloc(POISSON_TABLE_MAX * sizeof(uint32_t));
+  for (int k = 0; k < POISSON_TABLE_MAX; k++) g->lp_tab[k] = 0xFFFFFFFFu;
+  if (!(g->lp_lam >= 0.0f)) return -1;
+  return orc_poisson_table((double)g->lp_lam, g->lp_tab, POISSON_TABLE_MAX) < 0
+             ? -1 : 0;
 }
 
 int orc_net_n_var(orc_net *net, int group) { return net->groups[group].nvar; }
@@ -1270,6 +1289,22 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
           g->sg_next[i]++;
         }
       }
+    } else if (g->model == ORC_POISSON_LEGACY) {
+      /* src/poisson.cu:27-54: one count per node and step, dropped when
+       * above 5 lambda, pushed as the spike height.  The push precedes the
+       * time update of the step (src/neurongpu.cu:461-470), so the spike is
+       * stamped with the previous time index. */
+      uint32_t key[2] = {(uint32_t)net->seed, (uint32_t)(net->seed >> 32)};
+      for (int i = 0; i < g->n; i++) {
+        int node = g->i0 + i;
+        uint32_t ctr[4] = {(uint32_t)node, (uint32_t)T,
+                           (uint32_t)((uint64_t)T >> 32), 0x4C504F49u};
+        uint32_t out[4];
+        orc_philox4x32_10(ctr, key, out);
+        int k = orc_poisson_sample(g->lp_tab, out[0]);
+        if ((float)k > g->lp_lam * 5.0f) k = 0;
+        if (k > 0) push_spike(net, node, (REAL)k, T - 1, t);
+      }
     } else if (g->model == ORC_PARROT_NEURON) {
       /* src/parrot_neuron.cu:48-70 */
       for (int i = 0; i < g->n; i++) {
@@ -1451,6 +1486,9 @@ void orc_net_get_spikes(orc_net *net, long *step, int *node, int *count) {
     node[k] = net->spk[k].node;
     count[k] = net->spk[k].count;
   }
+}
+void orc_net_get_spike_heights(orc_net *net, double *height) {
+  for (long k = 0; k < net->n_spk; k++) height[k] = (double)net->spk[k].height;
 }
 int orc_net_n_records(orc_net *net) { return net->n_rec; }
 long orc_net_n_record_rows(orc_net *net) { return net->n_rec_rows; }

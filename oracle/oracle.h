@@ -27,6 +27,8 @@
  *   Poisson direct connections           src/poiss_gen.cu:43-74,
  *                                        src/neurongpu.cu:1156-1192
  *   spike generator                      src/spike_generator.cu:37-55, 209-238
+ *   CreatePoissonGenerator nodes         src/poisson.cu:27-54, 98-147,
+ *                                        src/neurongpu.cu:245-265, 461-470
  *
  * Deliberate restatement choices (shared with the HIP kernels):
  *   - reciprocals of Delta_T, C_m, tau_w, tau_rise, tau_decay are computed
@@ -56,7 +58,8 @@ enum {
   ORC_AEIF_PSC_ALPHA = 7,
   ORC_AEIF_PSC_EXP = 8,
   ORC_AEIF_PSC_DELTA = 9,
-  ORC_IZH_COND_BETA = 10
+  ORC_IZH_COND_BETA = 10,
+  ORC_POISSON_LEGACY = 11
 };
 
 /* ---- primitives ---- */
@@ -100,6 +103,9 @@ void orc_net_spike_gen(orc_net *net, int node, int n, const float *times_ms,
                        const float *heights);
 void orc_net_poisson(orc_net *net, int node, float rate, float origin,
                      float start, float stop);
+/* rate (Hz) of an ORC_POISSON_LEGACY group (CreatePoissonGenerator);
+ * returns -1 if the mean count per step is negative or above the table */
+int orc_net_legacy_rate(orc_net *net, int group, float rate);
 int orc_net_syn_group_stdp(orc_net *net, const float *param7);
 void orc_net_connect(orc_net *net, int n, const int *src, const int *tgt,
                      const uint8_t *port, const uint8_t *syn_group,
@@ -111,6 +117,8 @@ int orc_net_run(orc_net *net, long n_steps, int n_threads);
 long orc_net_n_spikes(orc_net *net);
 /* spike list: step index (0-based since calibration), node, multiplicity */
 void orc_net_get_spikes(orc_net *net, long *step, int *node, int *count);
+/* summed height of each entry of the spike list */
+void orc_net_get_spike_heights(orc_net *net, double *height);
 int orc_net_n_records(orc_net *net);
 long orc_net_n_record_rows(orc_net *net);
 void orc_net_get_records(orc_net *net, double *out); /* rows x (1+n_rec) */

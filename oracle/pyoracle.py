@@ -26,6 +26,7 @@ AEIF_PSC_ALPHA = 7
 AEIF_PSC_EXP = 8
 AEIF_PSC_DELTA = 9
 IZH_COND_BETA = 10
+POISSON_LEGACY = 11   # nodes of CreatePoissonGenerator
 
 MODEL_IDS = {
     "aeif_cond_beta": AEIF_COND_BETA,
@@ -41,6 +42,7 @@ MODEL_IDS = {
     # src/user_m1_kernel.h, src/user_m2_kernel.h: aeif_cond_beta renamed
     "user_m1": AEIF_COND_BETA,
     "user_m2": AEIF_COND_BETA,
+    "legacy_poisson_generator": POISSON_LEGACY,
 }
 
 AEIF_SCAL_PARAM = ["V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w",
@@ -91,6 +93,8 @@ def lib(prec="f32"):
         L.orc_net_set_rk5.argtypes = [vp, i, d, d, d]
         L.orc_net_spike_gen.argtypes = [vp, i, i, vp, vp]
         L.orc_net_poisson.argtypes = [vp, i, f, f, f, f]
+        L.orc_net_legacy_rate.restype = i
+        L.orc_net_legacy_rate.argtypes = [vp, i, f]
         L.orc_net_syn_group_stdp.argtypes = [vp, vp]
         L.orc_net_connect.argtypes = [vp, i, vp, vp, vp, vp, vp, vp]
         L.orc_net_record.argtypes = [vp, i, i, i]
@@ -99,6 +103,7 @@ def lib(prec="f32"):
         L.orc_net_n_spikes.restype = l
         L.orc_net_n_spikes.argtypes = [vp]
         L.orc_net_get_spikes.argtypes = [vp, vp, vp, vp]
+        L.orc_net_get_spike_heights.argtypes = [vp, vp]
         L.orc_net_n_records.argtypes = [vp]
         L.orc_net_n_record_rows.restype = l
         L.orc_net_n_record_rows.argtypes = [vp]
@@ -274,7 +279,8 @@ class Net:
         i0 = self.L.orc_net_add_group(self.h, MODEL_IDS[model], n, n_port)
         if model in ("iaf_psc_exp", "parrot_neuron"):
             n_port = 2
-        if model in ("spike_generator", "poisson_generator"):
+        if model in ("spike_generator", "poisson_generator",
+                     "legacy_poisson_generator"):
             n_port = 0
         g = Group(self, gid, model, i0, n, n_port)
         self.groups.append(g)
@@ -291,6 +297,11 @@ class Net:
 
     def poisson(self, node, rate, origin=0.0, start=0.0, stop=1.0e30):
         self.L.orc_net_poisson(self.h, node, rate, origin, start, stop)
+
+    def legacy_rate(self, g, rate):
+        """Rate (Hz) of a legacy_poisson_generator group."""
+        if self.L.orc_net_legacy_rate(self.h, g.gid, rate) != 0:
+            raise ValueError("legacy Poisson rate out of range")
 
     def stdp_group(self, **params):
         p = dict(STDP_DEFAULT)
@@ -341,6 +352,14 @@ class Net:
         if n:
             self.L.orc_net_get_spikes(self.h, _ptr(st), _ptr(nd), _ptr(ct))
         return st, nd, ct
+
+    def spike_heights(self):
+        """Summed height of each entry of spikes()."""
+        n = self.L.orc_net_n_spikes(self.h)
+        h = np.zeros(n, dtype=np.float64)
+        if n:
+            self.L.orc_net_get_spike_heights(self.h, _ptr(h))
+        return h
 
     def records(self):
         nr = self.L.orc_net_n_records(self.h)

@@ -5,6 +5,8 @@ same nodes, parameters, synapses and Poisson streams.
 A description is a dict:
   dt, seed, groups: [ {model, n, n_port, params{name: scalar|list|array},
                        vars{...}, rk5: {h_min_rel, h0_rel, rk5_tol}} ],
+          (model "legacy_poisson_generator" with "rate": the group of
+          CreatePoissonGenerator)
   conns: [ {src, tgt, weight, delay, port, syn_group} ]   (arrays, one entry
           per synapse, connected one_to_one in the given order)
   poisson: [ {node, rate, origin, start, stop} ]
@@ -25,6 +27,8 @@ def build_oracle(spec, prec="f32"):
     for g in spec["groups"]:
         og = net.create(g["model"], g["n"], g.get("n_port", 1))
         groups.append(og)
+        if g["model"] == "legacy_poisson_generator":
+            net.legacy_rate(og, g["rate"])
         for k, v in list(g.get("params", {}).items()) + list(
                 g.get("vars", {}).items()):
             og.set(k, v)
@@ -159,6 +163,24 @@ def brunel_spec(order=200, seed=1234, w_ex=0.05, w_in=0.35, w_p=0.37,
                        "weight": np.full(n, w_p, np.float32),
                        "delay": np.full(n, 0.2, np.float32), "port": 0}],
             "poisson": [{"node": n, "rate": rate}]}
+
+
+def legacy_brunel_spec(order=200, seed=4321, w_ex=0.0625, w_in=0.375,
+                       w_p=0.375, rate=20000.0, tol=1e-3):
+    """The network of the reference's c++/examples/test_error.cpp:24-110
+    (the Brunel network of brunel_spec, driven by CreatePoissonGenerator:
+    one generator node per neuron, connected one to one with delay 0.2 ms),
+    with dyadic weights so that every synaptic sum is exact."""
+    spec = brunel_spec(order=order, seed=seed, w_ex=w_ex, w_in=w_in,
+                       w_p=w_p, rate=rate, tol=tol)
+    n = 5 * order
+    spec["groups"][1] = {"model": "legacy_poisson_generator", "n": n,
+                         "rate": rate}
+    spec["conns"][2] = {"src": n + np.arange(n), "tgt": np.arange(n),
+                        "weight": np.full(n, w_p, np.float32),
+                        "delay": np.full(n, 0.2, np.float32), "port": 0}
+    spec["poisson"] = []
+    return spec
 
 
 def load_stdp_cases():

@@ -518,6 +518,40 @@ def test_izhikevich_trace_bit_exact(ngpu):
     assert np.array_equal(orc[:, 1:].astype(np.float32), data[:, 1:])
 
 
+@pytest.mark.parametrize("launch", [{"delivery_mode": 0},
+                                    {"delivery_mode": 1},
+                                    {"graph_steps": 16}])
+def test_legacy_poisson_network_bit_exact(ngpu, launch):
+    """Network of the reference's c++/examples/test_error.cpp (1000
+    aeif_cond_beta neurons, CreatePoissonGenerator at 20 kHz, one node per
+    neuron), dyadic weights, 100 ms: the neurons' and the generators' spike
+    trains equal the FP32 oracle's spike for spike.  Generator spikes carry
+    the time of the step before (src/neurongpu.cu:461-470)."""
+    log("case legacy_poisson_network", launch)
+    spec = nb.legacy_brunel_spec()
+    steps, n = 1000, 1000
+    groups, _ = nb.build_engine(ngpu, spec)
+    for k, v in launch.items():
+        ngpu.SetKernelStatus(k, v)
+    ngpu.ActivateRecSpikeTimes(groups[0], 2000)
+    ngpu.ActivateRecSpikeTimes(groups[1], steps + 10)
+    ngpu.Simulate(steps * 0.1)
+    eng = sorted(nb.engine_spikes(ngpu, groups[0]))
+    eng_gen = sorted(nb.engine_spikes(ngpu, groups[1]))
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(steps)
+    orc = sorted(nb.oracle_spikes(net, 0.1, 0, n))
+    st, nd, ct = net.spikes()
+    dt64 = np.float64(np.float32(0.1))
+    orc_gen = sorted((int(d), float(np.float32(dt64 * s)))
+                     for s, d in zip(st, nd) if d >= n)
+    log("spikes engine", len(eng), len(eng_gen), "oracle", len(orc),
+        len(orc_gen))
+    assert len(orc) > 100 and len(orc_gen) > 0.8 * n * steps
+    assert eng_gen == orc_gen
+    assert eng == orc
+
+
 @pytest.mark.parametrize("model", ["user_m1", "user_m2"])
 def test_user_models_match_aeif_cond_beta(ngpu, model):
     """user_m1 / user_m2 (aeif_cond_beta under other names in the

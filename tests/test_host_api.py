@@ -261,6 +261,32 @@ def test_error_unknown_model(ngpu):
         ngpu.Create("no_such_model", 3)
 
 
+def test_create_poisson_generator(ngpu):
+    """src/neurongpu.cu:245-265: one group, created before calibration,
+    with no receptor ports; a second call or n_node <= 0 is an error."""
+    a = ngpu.Create("aeif_cond_beta", 3, 2)
+    with pytest.raises(ValueError):
+        ngpu.CreatePoissonGenerator(0, 100.0)
+    pg = ngpu.CreatePoissonGenerator(4, 20000.0)
+    assert pg.i0 == 3 and pg.n == 4
+    with pytest.raises(ValueError):
+        ngpu.CreatePoissonGenerator(2, 100.0)
+    with pytest.raises(ValueError):     # no model of that name
+        ngpu.Create("base_neuron", 2)
+    with pytest.raises(ValueError):     # no receptor port on the generator
+        ngpu.Connect(a, pg, {"rule": "all_to_all"},
+                     {"weight": 1.0, "delay": 1.0})
+    ngpu.Connect(pg, a, {"rule": "all_to_all"}, {"weight": 0.5, "delay": 1.0})
+    assert len(ngpu.GetConnections(pg, a)) == 12
+
+
+def test_create_poisson_generator_rate_range(ngpu):
+    with pytest.raises(ValueError):     # mean count per step above 150
+        ngpu.CreatePoissonGenerator(4, 4.0e6)
+    with pytest.raises(ValueError):
+        ngpu.CreatePoissonGenerator(4, -5.0)
+
+
 def test_error_unknown_parameter(ngpu):
     a = ngpu.Create("aeif_cond_beta", 2)
     with pytest.raises(ValueError):

@@ -96,6 +96,95 @@ def test_poisson_counts_distribution():
 
 
 # ---------------------------------------------------------------------
+# CreatePoissonGenerator nodes (src/poisson.cu:27-54, src/neurongpu.cu:
+# 245-265): every step each node sends a spike of height n ~ Poisson(lambda)
+# unless n == 0 or n > 5 lambda, lambda = rate * dt / 1000.
+# ---------------------------------------------------------------------
+def _legacy_run(rate, n, steps, dt=0.1, seed=777):
+    net = po.Net(dt=dt, seed=seed)
+    g = net.create("legacy_poisson_generator", n)
+    net.legacy_rate(g, rate)
+    net.calibrate()
+    net.run(steps)
+    st, nd, ct = net.spikes()
+    return st, nd, ct, net.spike_heights()
+
+
+def test_legacy_poisson_heights_truncated_poisson():
+    """lambda = 2 (20 kHz at 0.1 ms): heights 1..10 with the Poisson pmf,
+    no spike with probability p(0) + p(k > 10) (chi-square, p > 1e-4)."""
+    n, steps = 500, 400
+    st, nd, ct, h = _legacy_run(20000.0, n, steps)
+    assert np.all(ct == 1)
+    assert np.all(h == np.round(h)) and h.min() >= 1 and h.max() <= 10
+    counts = np.bincount(h.astype(np.int64), minlength=11)[:11]
+    counts[0] = n * steps - len(h)
+    lam = float(np.float32(np.float32(20000.0) * np.float32(0.1)) / 1000.0)
+    pmf = scipy.stats.poisson.pmf(np.arange(11), lam)
+    pmf[0] += scipy.stats.poisson.sf(10, lam)
+    exp = pmf * n * steps
+    mask = exp > 5
+    chi2 = np.sum((counts[mask] - exp[mask]) ** 2 / exp[mask])
+    assert scipy.stats.chi2.sf(chi2, mask.sum() - 1) > 1e-4
+    # every (node, step) at most once, steps in range
+    key = st * n + (nd - nd.min())
+    assert len(np.unique(key)) == len(key)
+    assert st.min() >= 0 and st.max() < steps
+
+
+def test_legacy_poisson_cut_at_five_lambda():
+    """lambda = 0.2 (2 kHz): 5 lambda == 1 in float, so only counts of one
+    are kept (the cut is 'count > 5 lambda'); lambda = 0.1: 5 lambda < 1
+    and the generator never fires."""
+    n, steps = 1000, 300
+    st, nd, ct, h = _legacy_run(2000.0, n, steps)
+    assert np.all(h == 1.0)
+    p1 = 0.2 * math.exp(-0.2)
+    m = n * steps
+    z = (len(h) - m * p1) / math.sqrt(m * p1 * (1 - p1))
+    assert abs(z) < 5
+    st, nd, ct, h = _legacy_run(1000.0, n, steps)
+    assert len(h) == 0
+
+
+def test_legacy_poisson_rate_range():
+    net = po.Net(dt=0.1)
+    g = net.create("legacy_poisson_generator", 4)
+    with pytest.raises(ValueError):
+        net.legacy_rate(g, 4.0e6)      # lambda 400: table longer than 256
+    with pytest.raises(ValueError):
+        net.legacy_rate(g, -1.0)
+    net.legacy_rate(g, 0.0)
+
+
+def test_legacy_poisson_delivery_through_parrots():
+    """Spikes of height n reach a parrot neuron through a weight-1 synapse
+    of delay 0.5 ms as an input of n, two steps after the delay (one step
+    of the spike ring, one of the input fold), like any other spike."""
+    n, steps = 64, 200
+    net = po.Net(dt=0.1, seed=99)
+    g = net.create("legacy_poisson_generator", n)
+    net.legacy_rate(g, 20000.0)
+    par = net.create("parrot_neuron", n)
+    par.set("hold_spike_height", np.ones(n, np.float32))
+    idx = np.arange(n, dtype=np.int32)
+    net.connect(g.i0 + idx, par.i0 + idx, np.ones(n, np.float32),
+                np.full(n, 0.5, np.float32))
+    net.calibrate()
+    net.run(steps)
+    st, nd, ct = net.spikes()
+    h = net.spike_heights()
+    gen = {(int(s), int(d) - g.i0): v for s, d, v in zip(st, nd, h)
+           if d < par.i0}
+    out = {(int(s), int(d) - par.i0): v for s, d, v in zip(st, nd, h)
+           if d >= par.i0}
+    lag = 4 + 2
+    want = {(s + lag, i): v for (s, i), v in gen.items() if s + lag < steps}
+    assert len(want) > 1000
+    assert out == want
+
+
+# ---------------------------------------------------------------------
 # own exp / log (FP32, shared with the kernels)
 # ---------------------------------------------------------------------
 def _ulp_err(got, ref):

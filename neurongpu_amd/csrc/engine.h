@@ -34,6 +34,12 @@ enum ModelId {
   kIzhikevichCondBeta,
   kUserM1,          // src/user_m1_kernel.h: aeif_cond_beta under another name
   kUserM2,          // src/user_m2_kernel.h: the same
+  kIafPscExpG,      // fixed-step models of point_models.hip
+  kIafPscExpHc,
+  kIzhikevich,
+  kIzhikevichPscExp,
+  kIzhikevichPscExp2s,
+  kIzhikevichPscExp5s,
   kPoissonLegacy,   // CreatePoissonGenerator (src/poisson.cu); no model name
   kNumModels
 };
@@ -57,6 +63,19 @@ inline int Rk5KindOf(int m) {
   }
 }
 
+// the fixed-step models of point_models.hip: their PointKind, or -1
+inline int PointKindOf(int m) {
+  switch (m) {
+    case kIafPscExpG: return kPointIafG;
+    case kIafPscExpHc: return kPointIafHc;
+    case kIzhikevich: return kPointIzh;
+    case kIzhikevichPscExp:
+    case kIzhikevichPscExp2s:
+    case kIzhikevichPscExp5s: return kPointIzhPsc;
+    default: return -1;
+  }
+}
+
 struct ModelDesc {
   const char *name;
   std::vector<std::string> scal_var, port_var, scal_param, port_param,
@@ -68,6 +87,8 @@ struct ModelDesc {
 
 const ModelDesc &Model(ModelId id);
 int FindModel(const std::string &name);
+// iaf_psc_exp propagator P32 (engine.cpp)
+double IafP32(double tau_syn, double tau, double C, double h);
 // inversion thresholds of Poisson(lambda) (engine.cpp); returns the table
 // length, or -1 when max_len entries do not reach the top of the CDF
 int PoissonTable(double lambda, uint32_t *tab, int max_len);
@@ -129,6 +150,7 @@ struct Group {
   // kernel parameters (rebuilt when dirty)
   AeifParams aeif;
   IafArgs iaf;
+  PointArgs point;
   PField hold;
   float rk5_h_min = 0.f, rk5_h0 = 0.f, rk5_eps = 1e-6f;
   double rk5_h_min_acc = 0.0;

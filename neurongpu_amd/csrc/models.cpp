@@ -133,6 +133,52 @@ std::vector<ModelDesc> MakeModels() {
   m[kUserM2] = b;
   m[kUserM2].name = "user_m2";
 
+  // iaf_psc_exp_g (src/iaf_psc_exp_g.h:32-76, iaf_psc_exp_g.cu:96-118)
+  ModelDesc &ig = m[kIafPscExpG];
+  ig.name = "iaf_psc_exp_g";
+  ig.scal_var = {"I_syn", "V_m_rel", "refractory_step"};
+  ig.scal_var_default = {0.0f, 0.0f, 0.0f};
+  ig.scal_param = {"I_e"};
+  ig.scal_param_default = {0.0f};
+  ig.group_param = {"tau_m", "C_m", "E_L", "Theta_rel", "V_reset_rel",
+                    "tau_syn", "t_ref"};
+  ig.group_param_default = {10.0f, 250.0f, -65.0f, 15.0f, 0.0f, 0.5f, 2.0f};
+  ig.fixed_ports = 1;
+
+  // iaf_psc_exp_hc (src/iaf_psc_exp_hc.h:42-52): propagators fixed
+  ModelDesc &ih = m[kIafPscExpHc];
+  ih.name = "iaf_psc_exp_hc";
+  ih.scal_var = {"I_syn", "V_m_rel", "refractory_step"};
+  ih.scal_var_default = {0.0f, 0.0f, 0.0f};
+  ih.scal_param = {"I_e"};
+  ih.scal_param_default = {0.0f};
+  ih.fixed_ports = 1;
+
+  // izhikevich (src/izhikevich.h:54-73, izhikevich.cu:96-110)
+  ModelDesc &ie = m[kIzhikevich];
+  ie.name = "izhikevich";
+  ie.scal_var = {"I_syn", "V_m", "u", "refractory_step"};
+  ie.scal_var_default = {0.0f, -70.0f, -70.0f * 0.2f, 0.0f};
+  ie.scal_param = {"I_e", "den_delay"};
+  ie.scal_param_default = {0.0f, 0.0f};
+  ie.group_param = {"V_th", "a", "b", "c", "d", "t_ref"};
+  ie.group_param_default = {30.0f, 0.02f, 0.2f, -65.0f, 8.0f, 0.0f};
+  ie.fixed_ports = 1;
+
+  // izhikevich_psc_exp, _2s, _5s (src/izhikevich_psc_exp.h:56-76,
+  // izhikevich_psc_exp.cu:100-116)
+  const char *izp_name[3] = {"izhikevich_psc_exp", "izhikevich_psc_exp_2s",
+                             "izhikevich_psc_exp_5s"};
+  const ModelId izp_id[3] = {kIzhikevichPscExp, kIzhikevichPscExp2s,
+                             kIzhikevichPscExp5s};
+  for (int k = 0; k < 3; k++) {
+    ModelDesc &ip = m[izp_id[k]];
+    ip = ie;
+    ip.name = izp_name[k];
+    ip.group_param = {"V_th", "a", "b", "c", "d", "tau_syn", "t_ref"};
+    ip.group_param_default = {30.0f, 0.02f, 0.2f, -65.0f, 8.0f, 2.0f, 0.0f};
+  }
+
   // CreatePoissonGenerator nodes (src/neurongpu.cu:245-265): a BaseNeuron
   // group with no variables, parameters or receptor ports
   ModelDesc &lp = m[kPoissonLegacy];

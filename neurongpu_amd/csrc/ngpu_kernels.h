@@ -188,6 +188,31 @@ struct IafArgs {
   const StepClock *clock;
 };
 
+// Fixed-step point models with one input port folded into I_syn (weight 1)
+// and group-wide parameters (point_models.hip).
+enum PointKind {
+  kPointIafG = 0,    // iaf_psc_exp_g: exact integration, propagators below
+  kPointIafHc = 1,   // iaf_psc_exp_hc: the reference's fixed propagators
+  kPointIzh = 2,     // izhikevich: forward Euler, input added to V_m
+  kPointIzhPsc = 3   // izhikevich_psc_exp(_2s, _5s): Euler sub-steps, PSC
+};
+struct PointArgs {
+  int kind;
+  int n, i0;
+  int substeps;                 // izhikevich_psc_exp: 1, 2 or 5
+  float *I_syn, *V, *u, *refr;  // u: izhikevich models only
+  PField I_e;
+  float P11, P22, P21, P20;     // iaf_psc_exp_g
+  float Theta, V_reset;         // iaf_psc_exp_g (relative to E_L)
+  float V_th, a, b, c, d;       // izhikevich models
+  float h, C_syn;               // Euler sub-step and its PSC decay
+  int n_refr;                   // refractory steps set at a spike
+  FoldIn fold;
+  SpikeOut out;
+  int do_step;
+  const StepClock *clock;
+};
+
 // spike_detector (src/spike_detector.cu:46-68) and parrot_neuron
 // (src/parrot_neuron.cu:48-70): one input port folded into
 // input_spike_height.
@@ -337,6 +362,7 @@ const char *LaunchRk5Model(const Rk5ModelArgs &a, void *stream);
 void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
 void LaunchLegacyPoisson(const LegacyPoisArgs &a, void *stream);
+const char *LaunchPoint(const PointArgs &a, void *stream);
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
 void LaunchClockSet(StepClock *c, const StepClock &v, void *stream);
 // Binned delivery: count, scan, scatter, reduce, sum (five launches).

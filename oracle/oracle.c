@@ -628,6 +628,14 @@ static void model_sizes(int model, int n_port, int *nvar, int *nparam) {
   case ORC_AEIF_PSC_EXP: *nvar = 2 + n_port; *nparam = 14 + n_port; break;
   case ORC_AEIF_PSC_DELTA: *nvar = 2; *nparam = 14; break;
   case ORC_IZH_COND_BETA: *nvar = 2 + 2 * n_port; *nparam = 9 + 4 * n_port; break;
+  /* group parameters are stored as per-neuron columns after the scalar
+   * ones; the update reads them from neuron 0 */
+  case ORC_IAF_PSC_EXP_G: *nvar = 3; *nparam = 1 + 7; break;
+  case ORC_IAF_PSC_EXP_HC: *nvar = 3; *nparam = 1; break;
+  case ORC_IZHIKEVICH: *nvar = 4; *nparam = 2 + 6; break;
+  case ORC_IZH_PSC_EXP:
+  case ORC_IZH_PSC_EXP_2S:
+  case ORC_IZH_PSC_EXP_5S: *nvar = 4; *nparam = 2 + 7; break;
   default: *nvar = 0; *nparam = 0; break;
   }
 }
@@ -642,6 +650,7 @@ int orc_net_add_group(orc_net *net, int model, int n, int n_port) {
       model == ORC_POISSON_LEGACY)
     n_port = 0;
   if (model == ORC_PARROT_NEURON) n_port = 2;
+  if (model >= ORC_IAF_PSC_EXP_G && model <= ORC_IZH_PSC_EXP_5S) n_port = 1;
   g->n_port = n_port;
   g->i0 = net->n_node;
   model_sizes(model, n_port, &g->nvar, &g->nparam);
@@ -700,6 +709,24 @@ int orc_net_add_group(orc_net *net, int model, int n, int n_port) {
                     2.0f, 0.0f};
     for (int p = 0; p < 10; p++)
       for (int i = 0; i < n; i++) g->param[(size_t)p * n + i] = def[p];
+  } else if (model == ORC_IAF_PSC_EXP_G) {
+    /* src/iaf_psc_exp_g.cu:96-118 */
+    REAL def[7] = {10.0f, 250.0f, -65.0f, 15.0f, 0.0f, 0.5f, 2.0f};
+    for (int p = 0; p < 7; p++)
+      for (int i = 0; i < n; i++) g->param[(size_t)(1 + p) * n + i] = def[p];
+  } else if (model >= ORC_IZHIKEVICH && model <= ORC_IZH_PSC_EXP_5S) {
+    /* src/izhikevich.cu:96-110, src/izhikevich_psc_exp.cu:100-116 */
+    REAL def[7] = {30.0f, 0.02f, 0.2f, -65.0f, 8.0f, 2.0f, 0.0f};
+    int psc = model != ORC_IZHIKEVICH;
+    for (int p = 0, q = 0; p < 7; p++) {
+      if (!psc && p == 5) continue;   /* tau_syn */
+      for (int i = 0; i < n; i++) g->param[(size_t)(2 + q) * n + i] = def[p];
+      q++;
+    }
+    for (int i = 0; i < n; i++) {
+      g->var[(size_t)1 * n + i] = (REAL)-70.0f;
+      g->var[(size_t)2 * n + i] = (REAL)(-70.0f * 0.2f);
+    }
   } else if (model == ORC_SPIKE_GENERATOR) {
     g->sg_n = (int *)calloc(n, sizeof(int));
     g->sg_next = (int *)calloc(n, sizeof(int));
@@ -1199,6 +1226,133 @@ static int iaf_update_one(orc_net *net, orc_group *g, int i) {
   return 0;
 }
 
+/* Constants of one step of a fixed-step point model, as the reference's
+ * Update() forms them on the host in float (src/iaf_psc_exp_g.cu:136-150,
+ * src/izhikevich.cu:135-143, src/izhikevich_psc_exp_2s.cu:143-152). */
+typedef struct {
+  REAL P11, P22, P21, P20, Theta, V_reset;
+  REAL V_th, a, b, c, d, h, C_syn;
+  int n_refr, steps;
+} point_k;
+
+static void point_consts(orc_net *net, orc_group *g, point_k *k) {
+  memset(k, 0, sizeof(*k));
+  REAL dt = (REAL)net->dt;
+  if (g->model == ORC_IAF_PSC_EXP_G) {
+    REAL tau_m = *P(g, 1, 0), C_m = *P(g, 2, 0), tau_syn = *P(g, 6, 0);
+#if ORC_REAL_DOUBLE
+    k->P11 = exp(-dt / tau_syn);
+    k->P22 = exp(-dt / tau_m);
+#else
+    k->P11 = expf(-dt / tau_syn);
+    k->P22 = expf(-dt / tau_m);
+#endif
+    k->P21 = (REAL)orc_p32(tau_syn, tau_m, C_m, dt);
+    k->P20 = (REAL)((double)(tau_m / C_m) * (1.0 - (double)k->P22));
+    k->Theta = *P(g, 4, 0);
+    k->V_reset = *P(g, 5, 0);
+    k->n_refr = (int)ROUNDR(*P(g, 7, 0) / dt);
+  } else if (g->model >= ORC_IZHIKEVICH) {
+    k->V_th = *P(g, 2, 0);
+    k->a = *P(g, 3, 0);
+    k->b = *P(g, 4, 0);
+    k->c = *P(g, 5, 0);
+    k->d = *P(g, 6, 0);
+    k->steps = g->model == ORC_IZH_PSC_EXP_2S ? 2
+             : g->model == ORC_IZH_PSC_EXP_5S ? 5 : 1;
+    k->h = dt / (REAL)k->steps;
+    if (g->model == ORC_IZHIKEVICH) {
+      k->n_refr = (int)ROUNDR(*P(g, 7, 0) / k->h);
+    } else {
+#if ORC_REAL_DOUBLE
+      k->C_syn = exp(-k->h / *P(g, 7, 0));
+#else
+      k->C_syn = expf(-k->h / *P(g, 7, 0));
+#endif
+      k->n_refr = (int)ROUNDR(*P(g, 8, 0) / k->h);
+    }
+  }
+}
+
+/* 0.04 v v + 5 v + 140 - u in double, in the reference's order */
+static double izh_drive(REAL v, REAL u) {
+  double vd = (double)v;
+  double s = 0.04 * vd;
+  s = s * vd;
+  s = s + 5.0 * vd;
+  s = s + 140.0;
+  s = s - (double)u;
+  return s;
+}
+
+/* One step of a point model; returns 1 on a spike. */
+static int point_update_one(orc_group *g, const point_k *k, int i) {
+  REAL *I = V(g, 0, i), *Vm = V(g, 1, i);
+  REAL I_e = *P(g, 0, i);
+  if (g->model == ORC_IAF_PSC_EXP_G || g->model == ORC_IAF_PSC_EXP_HC) {
+    /* src/iaf_psc_exp_g.cu:34-61, src/iaf_psc_exp_hc.cu:29-53 */
+    REAL *refr = V(g, 2, i);
+    int hc = g->model == ORC_IAF_PSC_EXP_HC;
+    if (*refr > (REAL)0) {
+      *refr -= (REAL)1;
+    } else if (hc) {
+      /* src/iaf_psc_exp_hc_params.h: double constants */
+      double v = (double)*Vm * 9.9004983E-01;
+      v = v + (double)*I * 3.6067175E-04;
+      v = v + (double)I_e * 3.9800665E-04;
+      *Vm = (REAL)v;
+    } else {
+      REAL v = *Vm * k->P22;
+      v = FMA(*I, k->P21, v);
+      v = FMA(I_e, k->P20, v);
+      *Vm = v;
+    }
+    *I = hc ? (REAL)((double)*I * 8.1873075E-01) : *I * k->P11;
+    int fire = hc ? (double)*Vm >= 1.5E+01 : *Vm >= k->Theta;
+    if (fire) {
+      *Vm = hc ? (REAL)0 : k->V_reset;
+      *refr = (REAL)(hc ? 20 : k->n_refr);
+      return 1;
+    }
+    return 0;
+  }
+  /* izhikevich models: src/izhikevich.cu:35-72,
+   * src/izhikevich_psc_exp_2s.cu:41-83 */
+  REAL *u = V(g, 2, i), *refr = V(g, 3, i);
+  REAL ha = k->h * k->a;
+  if (g->model == ORC_IZHIKEVICH) {
+    if (*refr > (REAL)0) {
+      *refr -= (REAL)1;
+    } else {
+      REAL v0 = *Vm, u0 = *u;
+      double t = (double)k->h * (izh_drive(v0, u0) + (double)I_e);
+      t = t + (double)*I;
+      *Vm = (REAL)((double)*Vm + t);
+      *u = *u + ha * (k->b * v0 - u0);
+    }
+    *I = (REAL)0;
+  } else if (*refr > (REAL)0) {
+    *refr -= (REAL)1;
+    for (int s = 0; s < k->steps; s++) *I = *I * k->C_syn;
+  } else {
+    for (int s = 0; s < k->steps; s++) {
+      REAL v0 = *Vm, u0 = *u;
+      double r = izh_drive(v0, u0) + (double)*I;
+      r = r + (double)I_e;
+      *Vm = (REAL)((double)*Vm + (double)k->h * r);
+      *u = *u + ha * (k->b * v0 - u0);
+      *I = *I * k->C_syn;
+    }
+  }
+  if (*Vm >= k->V_th) {
+    *Vm = k->c;
+    *u = *u + k->d;
+    *refr = (REAL)(k->n_refr < 0 ? 0 : k->n_refr);
+    return 1;
+  }
+  return 0;
+}
+
 static void apply_stdp(orc_net *net, long c, float Dt) {
   int sg = net->conn[c].syn_group;
   net->conn[c].weight = orc_stdp_update(net->conn[c].weight, Dt,
@@ -1281,6 +1435,11 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
     } else if (g->model == ORC_IAF_PSC_EXP) {
       for (int i = 0; i < g->n; i++)
         if (iaf_update_one(net, g, i)) push_spike(net, g->i0 + i, (REAL)1, T, t);
+    } else if (g->model >= ORC_IAF_PSC_EXP_G && g->model <= ORC_IZH_PSC_EXP_5S) {
+      point_k k;
+      point_consts(net, g, &k);
+      for (int i = 0; i < g->n; i++)
+        if (point_update_one(g, &k, i)) push_spike(net, g->i0 + i, (REAL)1, T, t);
     } else if (g->model == ORC_SPIKE_GENERATOR) {
       for (int i = 0; i < g->n; i++) {
         int k = g->sg_next[i];
@@ -1422,6 +1581,12 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
           REAL *I = V(g, q, i);
           *I = *I + acc[(size_t)q * nn + g->i0 + i];
         }
+    } else if (g->model >= ORC_IAF_PSC_EXP_G && g->model <= ORC_IZH_PSC_EXP_5S) {
+      /* one port, weight 1, into I_syn */
+      for (int i = 0; i < g->n; i++) {
+        REAL *I = V(g, 0, i);
+        *I = *I + acc[g->i0 + i];
+      }
     } else if (g->model == ORC_PARROT_NEURON) {
       /* port 0 weight 1 into input_spike_height; port 1 weight 0 */
       for (int i = 0; i < g->n; i++) {

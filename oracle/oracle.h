@@ -27,6 +27,10 @@
  *   Poisson direct connections           src/poiss_gen.cu:43-74,
  *                                        src/neurongpu.cu:1156-1192
  *   spike generator                      src/spike_generator.cu:37-55, 209-238
+ *   iaf_psc_exp_g / _hc                  src/iaf_psc_exp_g.cu:34-61, 136-150,
+ *                                        src/iaf_psc_exp_hc.cu:29-53
+ *   izhikevich (Euler), _psc_exp(_2s,_5s) src/izhikevich.cu:35-72,
+ *                                        src/izhikevich_psc_exp_2s.cu:41-83
  *   CreatePoissonGenerator nodes         src/poisson.cu:27-54, 98-147,
  *                                        src/neurongpu.cu:245-265, 461-470
  *
@@ -59,7 +63,13 @@ enum {
   ORC_AEIF_PSC_EXP = 8,
   ORC_AEIF_PSC_DELTA = 9,
   ORC_IZH_COND_BETA = 10,
-  ORC_POISSON_LEGACY = 11
+  ORC_POISSON_LEGACY = 11,
+  ORC_IAF_PSC_EXP_G = 12,
+  ORC_IAF_PSC_EXP_HC = 13,
+  ORC_IZHIKEVICH = 14,
+  ORC_IZH_PSC_EXP = 15,
+  ORC_IZH_PSC_EXP_2S = 16,
+  ORC_IZH_PSC_EXP_5S = 17
 };
 
 /* ---- primitives ---- */

@@ -27,6 +27,12 @@ AEIF_PSC_EXP = 8
 AEIF_PSC_DELTA = 9
 IZH_COND_BETA = 10
 POISSON_LEGACY = 11   # nodes of CreatePoissonGenerator
+IAF_PSC_EXP_G = 12
+IAF_PSC_EXP_HC = 13
+IZHIKEVICH = 14
+IZH_PSC_EXP = 15
+IZH_PSC_EXP_2S = 16
+IZH_PSC_EXP_5S = 17
 
 MODEL_IDS = {
     "aeif_cond_beta": AEIF_COND_BETA,
@@ -43,7 +49,31 @@ MODEL_IDS = {
     "user_m1": AEIF_COND_BETA,
     "user_m2": AEIF_COND_BETA,
     "legacy_poisson_generator": POISSON_LEGACY,
+    "iaf_psc_exp_g": IAF_PSC_EXP_G,
+    "iaf_psc_exp_hc": IAF_PSC_EXP_HC,
+    "izhikevich": IZHIKEVICH,
+    "izhikevich_psc_exp": IZH_PSC_EXP,
+    "izhikevich_psc_exp_2s": IZH_PSC_EXP_2S,
+    "izhikevich_psc_exp_5s": IZH_PSC_EXP_5S,
 }
+
+# fixed-step point models: variables, then scalar + group parameters
+POINT_VARS = {
+    "iaf_psc_exp_g": ["I_syn", "V_m_rel", "refractory_step"],
+    "iaf_psc_exp_hc": ["I_syn", "V_m_rel", "refractory_step"],
+}
+POINT_PARAMS = {
+    "iaf_psc_exp_g": ["I_e", "tau_m", "C_m", "E_L", "Theta_rel",
+                      "V_reset_rel", "tau_syn", "t_ref"],
+    "iaf_psc_exp_hc": ["I_e"],
+    "izhikevich": ["I_e", "den_delay", "V_th", "a", "b", "c", "d", "t_ref"],
+}
+for _m in ("izhikevich", "izhikevich_psc_exp", "izhikevich_psc_exp_2s",
+           "izhikevich_psc_exp_5s"):
+    POINT_VARS[_m] = ["I_syn", "V_m", "u", "refractory_step"]
+    if _m != "izhikevich":
+        POINT_PARAMS[_m] = ["I_e", "den_delay", "V_th", "a", "b", "c", "d",
+                            "tau_syn", "t_ref"]
 
 AEIF_SCAL_PARAM = ["V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w",
                    "I_e", "V_peak", "V_reset", "t_ref", "refractory_step",
@@ -175,6 +205,8 @@ class Group:
             return IAF_VAR, []
         if self.model == "parrot_neuron":
             return ["input_spike_height", "dummy_input", "V"], []
+        if self.model in POINT_VARS:
+            return POINT_VARS[self.model], []
         return [], []
 
     def param_names(self):
@@ -194,6 +226,8 @@ class Group:
             return IAF_PARAM, []
         if self.model == "parrot_neuron":
             return ["hold_spike_height", "den_delay"], []
+        if self.model in POINT_PARAMS:
+            return POINT_PARAMS[self.model], []
         return [], []
 
     @staticmethod
@@ -279,6 +313,8 @@ class Net:
         i0 = self.L.orc_net_add_group(self.h, MODEL_IDS[model], n, n_port)
         if model in ("iaf_psc_exp", "parrot_neuron"):
             n_port = 2
+        if model in POINT_VARS:
+            n_port = 1
         if model in ("spike_generator", "poisson_generator",
                      "legacy_poisson_generator"):
             n_port = 0

@@ -264,6 +264,19 @@ def nest_spec(model):
                                   "delay": [d],
                                   "port": syn if n_port > 1 else 0})
         return spec
+    if model in ("iaf_psc_exp_g", "iaf_psc_exp_hc", "izhikevich"):
+        # python/test/test_iaf_psc_exp_g.py:8-27, test_iaf_psc_exp_hc.py,
+        # test_izh.py:5-24: one port, weights 1 / -2
+        izh = model == "izhikevich"
+        times, delays = ([10.0, 40.0], [1.0, 10.0]) if izh else \
+            ([10.0, 400.0], [1.0, 100.0])
+        return {"groups": [{"model": model, "n": 1},
+                           {"model": "spike_generator", "n": 1}],
+                "spike_gen": [{"node": 1, "times": times}],
+                "conns": [{"src": [1], "tgt": [0], "weight": [w],
+                           "delay": [d], "port": 0}
+                          for w, d in zip([1.0, -2.0], delays)],
+                "records": [(0, "V_m" if izh else "V_m_rel", 0)]}
     if model == "iaf_psc_exp":
         return {"groups": [{"model": "iaf_psc_exp", "n": 1},
                            {"model": "spike_generator", "n": 1}],

@@ -518,6 +518,76 @@ def test_izhikevich_trace_bit_exact(ngpu):
     assert np.array_equal(orc[:, 1:].astype(np.float32), data[:, 1:])
 
 
+# python/test/test_iaf_psc_exp_g.py, test_iaf_psc_exp_hc.py, test_izh.py
+POINT_NEST = {"iaf_psc_exp_g": ("fast_iaf_psc_exp_nest.txt", 800.0, -65.0, 5e-5),
+              "iaf_psc_exp_hc": ("fast_iaf_psc_exp_nest.txt", 800.0, -65.0, 5e-5),
+              "izhikevich": ("izhikevich_nest.txt", 80.0, 0.0, 5e-3)}
+
+
+@pytest.mark.parametrize("model", list(POINT_NEST))
+def test_point_model_nest_trace(ngpu, model):
+    """The reference's golden NEST traces of iaf_psc_exp_g / _hc (5e-5)
+    and izhikevich (its tolerance 5e-3); the trace equals the FP32
+    oracle's value for value."""
+    log("case point_nest", model)
+    fname, t_sim, E_L, tol = POINT_NEST[model]
+    spec = nb.nest_spec(model)
+    _, recs = nb.build_engine(ngpu, spec)
+    ngpu.Simulate(t_sim)
+    data = ngpu.GetRecordDataArray(recs[0])
+    V = data[:, 1].astype(np.float64) + E_L
+    nest = np.loadtxt(os.path.join(GOLDEN, fname), delimiter="\t")
+    dV = np.array([V[i * 10 + 20] - nest[i, 1] for i in range(len(nest))])
+    rmse = np.std(dV) / abs(np.mean(V))
+    log("rmse", rmse, "tol", tol)
+    assert rmse <= tol
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(int(round(t_sim / 0.1)))
+    orc = net.records()
+    assert orc.shape == data.shape
+    assert np.array_equal(orc[:, 1].astype(np.float32), data[:, 1])
+
+
+def _point_net_spec(model, n=500, k=40, seed=11):
+    """Recurrent network of one fixed-step model, 80 % excitatory, driven
+    by per-neuron I_e; integer weights so that every input sum is exact."""
+    rng = np.random.default_rng(seed)
+    n_e = int(0.8 * n)
+    params = {}
+    if model.startswith("iaf"):
+        params["I_e"] = rng.uniform(360.0, 420.0, n).astype(np.float32)
+        w_e, w_i = 64.0, -256.0
+    elif model == "izhikevich":
+        params["I_e"] = rng.uniform(4.0, 12.0, n).astype(np.float32)
+        w_e, w_i = 1.0, -4.0
+    else:
+        params["I_e"] = rng.uniform(4.0, 12.0, n).astype(np.float32)
+        w_e, w_i = 2.0, -8.0
+    src = rng.integers(0, n, n * k)
+    tgt = np.repeat(np.arange(n), k)
+    w = np.where(src < n_e, w_e, w_i).astype(np.float32)
+    d = rng.integers(10, 41, n * k) * 0.1
+    return {"groups": [{"model": model, "n": n, "params": params}],
+            "conns": [{"src": src, "tgt": tgt, "weight": w, "delay": d,
+                       "port": 0}],
+            "records": []}
+
+
+@pytest.mark.parametrize("model", ["iaf_psc_exp_g", "iaf_psc_exp_hc",
+                                   "izhikevich", "izhikevich_psc_exp",
+                                   "izhikevich_psc_exp_2s",
+                                   "izhikevich_psc_exp_5s"])
+def test_point_model_network_bit_exact(ngpu, model):
+    """500-neuron recurrent network of each fixed-step model, 200 ms:
+    spike trains equal the FP32 oracle's spike for spike."""
+    log("case point_network", model)
+    spec = _point_net_spec(model)
+    eng, orc, net = _run_both(ngpu, spec, 2000, 500)
+    log("spikes engine", len(eng), "oracle", len(orc))
+    assert len(orc) > 200
+    assert eng == orc
+
+
 @pytest.mark.parametrize("launch", [{"delivery_mode": 0},
                                     {"delivery_mode": 1},
                                     {"graph_steps": 16}])

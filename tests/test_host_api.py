@@ -261,6 +261,40 @@ def test_error_unknown_model(ngpu):
         ngpu.Create("no_such_model", 3)
 
 
+@pytest.mark.parametrize("model,defaults,group", [
+    # src/iaf_psc_exp_g.cu:96-118
+    ("iaf_psc_exp_g", {"I_syn": 0.0, "V_m_rel": 0.0, "I_e": 0.0,
+                       "tau_m": 10.0, "C_m": 250.0, "E_L": -65.0,
+                       "Theta_rel": 15.0, "tau_syn": 0.5, "t_ref": 2.0},
+     ["tau_m", "C_m", "E_L", "Theta_rel", "V_reset_rel", "tau_syn", "t_ref"]),
+    # src/iaf_psc_exp_hc.cu:72-84
+    ("iaf_psc_exp_hc", {"I_syn": 0.0, "V_m_rel": 0.0, "I_e": 0.0}, []),
+    # src/izhikevich.cu:96-110
+    ("izhikevich", {"V_m": -70.0, "u": -14.0, "V_th": 30.0, "c": -65.0,
+                    "d": 8.0, "t_ref": 0.0, "den_delay": 0.0},
+     ["V_th", "a", "b", "c", "d", "t_ref"]),
+    # src/izhikevich_psc_exp_5s.cu:110-122
+    ("izhikevich_psc_exp_5s", {"V_m": -70.0, "u": -14.0, "tau_syn": 2.0},
+     ["V_th", "a", "b", "c", "d", "tau_syn", "t_ref"]),
+])
+def test_point_model_defaults(ngpu, model, defaults, group):
+    """Fixed-step models: the reference's defaults and group parameters,
+    one receptor port whatever n_port asks for."""
+    ns = ngpu.Create(model, 3, 4)
+    st = ngpu.GetStatus(ns[2])
+    for k, v in defaults.items():
+        assert st[k] == pytest.approx(v, rel=1e-7), k
+    assert ngpu.GetGroupParamNames(ns[0]) == group
+    with pytest.raises(ValueError):     # port 1 does not exist
+        ngpu.Connect(ns, ns, {"rule": "all_to_all"},
+                     {"weight": 1.0, "delay": 1.0, "receptor": 1})
+    if group:
+        ngpu.SetStatus(ns, {group[0]: 1.5})
+        assert ngpu.GetStatus(ns[1], group[0]) == 1.5
+        with pytest.raises(ValueError):  # a group parameter of part of it
+            ngpu.SetNeuronGroupParam(ngpu.NodeSeq(ns.i0, 2), group[0], 1.0)
+
+
 def test_create_poisson_generator(ngpu):
     """src/neurongpu.cu:245-265: one group, created before calibration,
     with no receptor ports; a second call or n_node <= 0 is an error."""

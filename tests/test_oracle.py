@@ -359,6 +359,122 @@ def test_nest_golden_trace_oracle(model, prec):
     assert rmse <= tol
 
 
+# python/test/test_iaf_psc_exp_g.py and test_iaf_psc_exp_hc.py (both against
+# test_fast_iaf_psc_exp_nest.txt, 800 ms; tolerance 5e-5, tighter than the
+# reference's 5e-4), and test_izh.py (test_izh_nest.txt, 80 ms; the
+# reference's tolerance 5e-3)
+POINT_NEST = {"iaf_psc_exp_g": ("fast_iaf_psc_exp_nest.txt", 8000, -65.0, 5e-5),
+              "iaf_psc_exp_hc": ("fast_iaf_psc_exp_nest.txt", 8000, -65.0, 5e-5),
+              "izhikevich": ("izhikevich_nest.txt", 800, 0.0, 5e-3)}
+
+
+def point_nest_rmse(V, model):
+    fname, _, E_L, _ = POINT_NEST[model]
+    V = V + E_L
+    nest = np.loadtxt(os.path.join(GOLDEN, fname), delimiter="\t")
+    dV = np.array([V[i * 10 + 20] - nest[i, 1] for i in range(len(nest))])
+    return np.std(dV) / abs(np.mean(V))
+
+
+@pytest.mark.parametrize("model", list(POINT_NEST))
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_point_model_nest_trace_oracle(model, prec):
+    spec = nb.nest_spec(model)
+    net, _ = nb.build_oracle(spec, prec)
+    net.run(POINT_NEST[model][1])
+    V = net.records()[:, 1]
+    assert point_nest_rmse(V, model) <= POINT_NEST[model][3]
+
+
+def test_iaf_psc_exp_hc_constants():
+    """iaf_psc_exp_hc's fixed propagators (src/iaf_psc_exp_hc_params.h) are
+    those of iaf_psc_exp_g's defaults: tau_m 10, C_m 250, tau_syn 0.5,
+    t_ref 2 at dt 0.1 (scipy expm of the linear system, 8 digits)."""
+    h, tau_m, C_m, tau_s = 0.1, 10.0, 250.0, 0.5
+    A = np.array([[-1.0 / tau_s, 0.0], [1.0 / C_m, -1.0 / tau_m]])
+    E = scipy.linalg.expm(A * h)
+    P11, P21, P22 = E[0, 0], E[1, 0], E[1, 1]
+    P20 = tau_m / C_m * (1.0 - P22)
+    for got, want in ((8.1873075E-01, P11), (3.6067175E-04, P21),
+                      (9.9004983E-01, P22), (3.9800665E-04, P20)):
+        assert abs(got - want) <= 5e-8 * abs(want) + 1e-15
+    # and the g model with those parameters gives the hc trace to float
+    # rounding
+    sg = nb.nest_spec("iaf_psc_exp_g")
+    sh = nb.nest_spec("iaf_psc_exp_hc")
+    ng, _ = nb.build_oracle(sg, "f32")
+    nh, _ = nb.build_oracle(sh, "f32")
+    ng.run(8000)
+    nh.run(8000)
+    assert np.max(np.abs(ng.records()[:, 1] - nh.records()[:, 1])) < 1e-4
+
+
+def _izh_psc_run(model, dt, t_end, I_e, w, tau_syn, prec="f64"):
+    spec = {"dt": dt,
+            "groups": [{"model": model, "n": 1,
+                        "params": {"I_e": I_e, "tau_syn": tau_syn},
+                        "vars": {"I_syn": w}}],
+            "records": [(0, "V_m", 0), (0, "I_syn", 0)]}
+    net, _ = nb.build_oracle(spec, prec)
+    net.run(int(round(t_end / dt)))
+    return net.records()
+
+
+def test_izhikevich_psc_exp_euler_order():
+    """izhikevich_psc_exp, _2s, _5s are forward Euler on 1, 2, 5 sub-steps
+    of dt for dV = 0.04 V^2 + 5 V + 140 - u + I_syn + I_e, du = a (b V - u),
+    dI_syn = -I_syn / tau_syn.  From V = -70, u = -14, I_syn = 2 (below
+    threshold), the V_m error against scipy LSODA shrinks in proportion to
+    the sub-step: first order."""
+    import scipy.integrate
+    I_e, w, tau_syn, t_end = 1.0, 2.0, 2.0, 40.0
+
+    def f(t, y):
+        V, u, I = y
+        return [0.04 * V * V + 5 * V + 140 - u + I + I_e,
+                0.02 * (0.2 * V - u), -I / tau_syn]
+    sol = scipy.integrate.solve_ivp(f, (0, t_end), [-70.0, -14.0, w],
+                                    method="LSODA", rtol=1e-12, atol=1e-12,
+                                    dense_output=True).sol
+    errs = {}
+    for model, k in (("izhikevich_psc_exp", 1), ("izhikevich_psc_exp_2s", 2),
+                     ("izhikevich_psc_exp_5s", 5)):
+        r = _izh_psc_run(model, 0.1, t_end, I_e, w, tau_syn)
+        assert r[:, 1].max() < -60.0
+        errs[k] = np.max(np.abs(r[:, 1] - sol(r[:, 0])[0]))
+    assert 1.8 < errs[1] / errs[2] < 2.2
+    assert 4.5 < errs[1] / errs[5] < 5.5
+    assert errs[5] < 0.02
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_izhikevich_psc_exp_substeps(prec):
+    """_2s at dt takes the same sub-steps as the one-step model at dt/2
+    (the current decays after every sub-step): V_m and I_syn agree bit for
+    bit at the common times."""
+    a = _izh_psc_run("izhikevich_psc_exp_2s", 0.1, 20.0, 1.0, 2.0, 2.0, prec)
+    b = _izh_psc_run("izhikevich_psc_exp", 0.05, 20.0, 1.0, 2.0, 2.0, prec)
+    assert np.array_equal(a[:, 1:], b[::2, 1:])
+
+
+def test_izhikevich_spike_reset():
+    """Suprathreshold current: each crossing of V_th = 30 resets V_m to c
+    and adds d to u (src/izhikevich.cu:57-68); t_ref = 0 leaves no
+    refractory steps."""
+    spec = {"groups": [{"model": "izhikevich", "n": 1,
+                        "params": {"I_e": 10.0}}],
+            "records": [(0, "V_m", 0), (0, "u", 0)]}
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(3000)
+    st, nd, ct = net.spikes()
+    r = net.records()
+    assert len(st) >= 5
+    for s in st[:5]:
+        # record row s + 1 is the end of the spiking step
+        assert r[s + 1, 1] == np.float32(-65.0)
+        assert r[s + 1, 2] > r[s, 2] + 7.0         # u jumped by d = 8
+
+
 def test_aeif_fixed_point():
     """Constant I_e below rheobase: V, w converge to the root of
     dV = dw = 0 with w* = a (V* - E_L)."""

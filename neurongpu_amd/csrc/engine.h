@@ -40,6 +40,7 @@ enum ModelId {
   kIzhikevichPscExp,
   kIzhikevichPscExp2s,
   kIzhikevichPscExp5s,
+  kExtNeuron,       // src/ext_neuron.cu
   kPoissonLegacy,   // CreatePoissonGenerator (src/poisson.cu); no model name
   kNumModels
 };
@@ -338,6 +339,9 @@ class Engine {
   float legacy_lambda = 0.0f;
   std::vector<uint32_t> legacy_tab;
   DBuf<uint32_t> d_legacy_tab;
+  // the last GetExtNeuronInputSpikes result (library-owned arrays)
+  std::vector<int> ext_node, ext_port;
+  std::vector<float> ext_height;
   // statistics
   DBuf<unsigned long long> d_cnt_rk5, d_cnt_events;
   unsigned long long n_spikes_total = 0;

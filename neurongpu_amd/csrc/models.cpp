@@ -179,6 +179,17 @@ std::vector<ModelDesc> MakeModels() {
     ip.group_param_default = {30.0f, 0.02f, 0.2f, -65.0f, 8.0f, 2.0f, 0.0f};
   }
 
+  // ext_neuron (src/ext_neuron.h:46-60, ext_neuron.cu:45-84)
+  ModelDesc &ex = m[kExtNeuron];
+  ex.name = "ext_neuron";
+  ex.port_var = {"port_input", "port_value"};
+  ex.port_var_default = {0.0f, 0.0f};
+  ex.scal_param = {"den_delay"};
+  ex.scal_param_default = {0.0f};
+  ex.port_param = {"port_weight"};
+  ex.port_param_default = {1.0f};
+  ex.fixed_ports = -1;
+
   // CreatePoissonGenerator nodes (src/neurongpu.cu:245-265): a BaseNeuron
   // group with no variables, parameters or receptor ports
   ModelDesc &lp = m[kPoissonLegacy];

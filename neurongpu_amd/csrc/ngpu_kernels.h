@@ -213,6 +213,17 @@ struct PointArgs {
   const StepClock *clock;
 };
 
+// ext_neuron (src/ext_neuron.cu:23-39): per port, the input folded with
+// port_weight is moved to port_value at each step, for the host to read.
+struct ExtArgs {
+  int n, i0, n_port;
+  float *var;                   // [2 n_port][n]: port_input, port_value
+  const float *port_weight;     // [n_port][n]
+  FoldIn fold;
+  int do_step;
+  const StepClock *clock;
+};
+
 // spike_detector (src/spike_detector.cu:46-68) and parrot_neuron
 // (src/parrot_neuron.cu:48-70): one input port folded into
 // input_spike_height.
@@ -363,6 +374,7 @@ void LaunchRelay(const RelayArgs &a, void *stream);
 void LaunchSpikeGen(const SpikeGenArgs &a, void *stream);
 void LaunchLegacyPoisson(const LegacyPoisArgs &a, void *stream);
 const char *LaunchPoint(const PointArgs &a, void *stream);
+void LaunchExt(const ExtArgs &a, void *stream);
 void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
 void LaunchClockSet(StepClock *c, const StepClock &v, void *stream);
 // Binned delivery: count, scan, scatter, reduce, sum (five launches).

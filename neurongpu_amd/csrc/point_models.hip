@@ -11,9 +11,11 @@
 //                   src/izhikevich_psc_exp.cu:40-76,
 //                   src/izhikevich_psc_exp_2s.cu:41-83 (forward Euler on 1,
 //                   2 or 5 sub-steps of dt, exponentially decaying current)
+//   ext_neuron      src/ext_neuron.cu:23-39 (per-port input moved to
+//                   port_value for the host; no spikes)
 //
-// Each has one input port whose accumulator is added to I_syn with weight 1
-// before the update.  The izhikevich membrane equations carry the
+// The iaf and izhikevich models have one input port whose accumulator is
+// added to I_syn with weight 1 before the update.  The izhikevich membrane equations carry the
 // reference's double literals, so they are evaluated in double with the
 // reference's operation order; everything else is FP32.
 #include <hip/hip_runtime.h>
@@ -136,6 +138,23 @@ __global__ __launch_bounds__(kBlock) void point_kernel(PointArgs A) {
   emit_spikes(A.out, sn, active, ii, node, nsp, 1.0f);
 }
 
+// ext_neuron, one thread per (node, port), nodes fastest
+__global__ __launch_bounds__(kBlock) void ext_kernel(ExtArgs A) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)A.n * A.n_port) return;
+  const int p = (int)(t / A.n), i = (int)(t % A.n);
+  const long long pois_T = A.clock ? A.clock->pois_T : A.fold.pois_T;
+  const double pois_t = A.clock ? A.clock->pois_t : A.fold.pois_t;
+  const float in = gather_port(A.fold, pois_T, pois_t, A.i0 + i, p);
+  float *pin = A.var + (size_t)(2 * p) * A.n + i;
+  float x = __builtin_fmaf(in, A.port_weight[(size_t)p * A.n + i], *pin);
+  if (A.do_step) {
+    A.var[(size_t)(2 * p + 1) * A.n + i] = x;
+    x = 0.0f;
+  }
+  *pin = x;
+}
+
 template <int KIND, int STEPS>
 void launch(const PointArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((point_kernel<KIND, STEPS>),
@@ -159,6 +178,13 @@ const char *LaunchPoint(const PointArgs &a, void *stream) {
     default:
       return "unknown point model";
   }
+}
+
+void LaunchExt(const ExtArgs &a, void *stream) {
+  long long total = (long long)a.n * a.n_port;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(ext_kernel, dim3((unsigned)((total + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, (hipStream_t)stream, a);
 }
 
 }  // namespace ngpu

@@ -130,6 +130,36 @@ __device__ __forceinline__ void gather_input(const FoldIn &f,
   }
 }
 
+// One port of gather_input, for models whose port count is known only at
+// run time: the accumulator of port p, then the Poisson draws of the node's
+// direct connections to that port in canonical order.
+__device__ __forceinline__ float gather_port(const FoldIn &f, long long pois_T,
+                                             double pois_t, int node, int p) {
+  float *ap = f.acc + (long long)p * f.n_total + node;
+  float a = *ap;
+  *ap = 0.0f;
+  if (pois_T < 0 || !f.dir.active) return a;
+  const DirectIn &d = f.dir;
+  uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
+  uint32_t Tlo = (uint32_t)pois_T, Thi = (uint32_t)((uint64_t)pois_T >> 32);
+  for (uint32_t j = j0; j < j1; j++) {
+    if ((int)d.port[j] != p) continue;
+    uint32_t g = d.gen[j];
+    double t_rel = pois_t - (double)d.gen_origin[g] - (double)d.delay[j];
+    if (!(t_rel >= (double)d.gen_start[g] && t_rel <= (double)d.gen_stop[g]))
+      continue;
+    uint32_t ti = d.gen_table[g];
+    if (ti == 0xFFFFFFFFu) continue;
+    const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
+    uint32_t u = dev_philox_w0(d.canon[j], Tlo, Thi, kPoissonTag, d.key0,
+                               d.key1);
+    int k = 0;
+    while (u > tab[k]) k++;
+    if (k > 0) a = a + d.weight[j] * (float)k;
+  }
+  return a;
+}
+
 // Start of an adaptive Cash-Karp step (src/rk5.h:57-69) of model M (M::N
 // state variables, m.rhs(refr, y, dy) the derivatives): first stage and
 // the error scale, which stay fixed over the step's retries.

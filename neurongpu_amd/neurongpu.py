@@ -1293,6 +1293,21 @@ def PushSpikesToNodes(nodes):
     return _call("PushSpikesToNodes", a.size, p)
 
 
+def GetExtNeuronInputSpikes(include_zeros=False):
+    """(node, port, height) lists of the ext_neuron ports whose
+    port_value is non-zero (all ports with include_zeros), through
+    NeuronGPU_GetExtNeuronInputSpikes (src/neurongpu_C.h:288-289)."""
+    n = ctypes.c_int(0)
+    pn = ctypes.POINTER(ctypes.c_int)()
+    pp = ctypes.POINTER(ctypes.c_int)()
+    ph = ctypes.POINTER(ctypes.c_float)()
+    _call("GetExtNeuronInputSpikes", ctypes.byref(n), ctypes.byref(pn),
+          ctypes.byref(pp), ctypes.byref(ph), 1 if include_zeros else 0)
+    k = n.value
+    return ([pn[i] for i in range(k)], [pp[i] for i in range(k)],
+            [ph[i] for i in range(k)])
+
+
 # ---------------------------------------------------------------------
 # kernel parameters
 def GetNFloatParam():

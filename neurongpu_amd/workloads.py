@@ -19,7 +19,8 @@ streams.
 """
 import numpy as np
 
-PORT_PARAMS = {"E_rev", "tau_rise", "tau_decay", "tau_syn", "g0"}
+PORT_PARAMS = {"E_rev", "tau_rise", "tau_decay", "tau_syn", "g0",
+               "port_weight"}
 
 
 def build_from_spec(ngpu, spec):
@@ -75,7 +76,7 @@ def build_from_spec(ngpu, spec):
 
 
 def _set(ngpu, ns, k, v):
-    if k in PORT_PARAMS:
+    if k in PORT_PARAMS and np.ndim(v) > 0:
         ngpu.SetStatus(ns, {k: list(map(float, v))})
     elif np.ndim(v) == 0:
         ngpu.SetStatus(ns, {k: float(v)})

@@ -636,6 +636,8 @@ static void model_sizes(int model, int n_port, int *nvar, int *nparam) {
   case ORC_IZH_PSC_EXP:
   case ORC_IZH_PSC_EXP_2S:
   case ORC_IZH_PSC_EXP_5S: *nvar = 4; *nparam = 2 + 7; break;
+  /* port_input, port_value per port; den_delay, port_weight per port */
+  case ORC_EXT_NEURON: *nvar = 2 * n_port; *nparam = 1 + n_port; break;
   default: *nvar = 0; *nparam = 0; break;
   }
 }
@@ -709,6 +711,10 @@ int orc_net_add_group(orc_net *net, int model, int n, int n_port) {
                     2.0f, 0.0f};
     for (int p = 0; p < 10; p++)
       for (int i = 0; i < n; i++) g->param[(size_t)p * n + i] = def[p];
+  } else if (model == ORC_EXT_NEURON) {
+    /* port_weight 1 (src/ext_neuron.cu:71-77) */
+    for (int q = 0; q < n_port; q++)
+      for (int i = 0; i < n; i++) g->param[(size_t)(1 + q) * n + i] = (REAL)1;
   } else if (model == ORC_IAF_PSC_EXP_G) {
     /* src/iaf_psc_exp_g.cu:96-118 */
     REAL def[7] = {10.0f, 250.0f, -65.0f, 15.0f, 0.0f, 0.5f, 2.0f};
@@ -1435,6 +1441,13 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
     } else if (g->model == ORC_IAF_PSC_EXP) {
       for (int i = 0; i < g->n; i++)
         if (iaf_update_one(net, g, i)) push_spike(net, g->i0 + i, (REAL)1, T, t);
+    } else if (g->model == ORC_EXT_NEURON) {
+      /* src/ext_neuron.cu:23-39: port_value = port_input; port_input = 0 */
+      for (int q = 0; q < g->n_port; q++)
+        for (int i = 0; i < g->n; i++) {
+          *V(g, 2 * q + 1, i) = *V(g, 2 * q, i);
+          *V(g, 2 * q, i) = (REAL)0;
+        }
     } else if (g->model >= ORC_IAF_PSC_EXP_G && g->model <= ORC_IZH_PSC_EXP_5S) {
       point_k k;
       point_consts(net, g, &k);
@@ -1580,6 +1593,13 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
         for (int i = 0; i < g->n; i++) {
           REAL *I = V(g, q, i);
           *I = *I + acc[(size_t)q * nn + g->i0 + i];
+        }
+    } else if (g->model == ORC_EXT_NEURON) {
+      /* port_input += input * port_weight */
+      for (int q = 0; q < g->n_port; q++)
+        for (int i = 0; i < g->n; i++) {
+          REAL *in = V(g, 2 * q, i);
+          *in = FMA(acc[(size_t)q * nn + g->i0 + i], *P(g, 1 + q, i), *in);
         }
     } else if (g->model >= ORC_IAF_PSC_EXP_G && g->model <= ORC_IZH_PSC_EXP_5S) {
       /* one port, weight 1, into I_syn */

@@ -31,6 +31,7 @@
  *                                        src/iaf_psc_exp_hc.cu:29-53
  *   izhikevich (Euler), _psc_exp(_2s,_5s) src/izhikevich.cu:35-72,
  *                                        src/izhikevich_psc_exp_2s.cu:41-83
+ *   ext_neuron                           src/ext_neuron.cu:23-39
  *   CreatePoissonGenerator nodes         src/poisson.cu:27-54, 98-147,
  *                                        src/neurongpu.cu:245-265, 461-470
  *
@@ -69,7 +70,8 @@ enum {
   ORC_IZHIKEVICH = 14,
   ORC_IZH_PSC_EXP = 15,
   ORC_IZH_PSC_EXP_2S = 16,
-  ORC_IZH_PSC_EXP_5S = 17
+  ORC_IZH_PSC_EXP_5S = 17,
+  ORC_EXT_NEURON = 18
 };
 
 /* ---- primitives ---- */

@@ -33,6 +33,7 @@ IZHIKEVICH = 14
 IZH_PSC_EXP = 15
 IZH_PSC_EXP_2S = 16
 IZH_PSC_EXP_5S = 17
+EXT_NEURON = 18
 
 MODEL_IDS = {
     "aeif_cond_beta": AEIF_COND_BETA,
@@ -55,6 +56,7 @@ MODEL_IDS = {
     "izhikevich_psc_exp": IZH_PSC_EXP,
     "izhikevich_psc_exp_2s": IZH_PSC_EXP_2S,
     "izhikevich_psc_exp_5s": IZH_PSC_EXP_5S,
+    "ext_neuron": EXT_NEURON,
 }
 
 # fixed-step point models: variables, then scalar + group parameters
@@ -207,6 +209,8 @@ class Group:
             return ["input_spike_height", "dummy_input", "V"], []
         if self.model in POINT_VARS:
             return POINT_VARS[self.model], []
+        if self.model == "ext_neuron":
+            return [], ["port_input", "port_value"]
         return [], []
 
     def param_names(self):
@@ -228,6 +232,8 @@ class Group:
             return ["hold_spike_height", "den_delay"], []
         if self.model in POINT_PARAMS:
             return POINT_PARAMS[self.model], []
+        if self.model == "ext_neuron":
+            return ["den_delay"], ["port_weight"]
         return [], []
 
     @staticmethod

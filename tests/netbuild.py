@@ -183,6 +183,31 @@ def legacy_brunel_spec(order=200, seed=4321, w_ex=0.0625, w_in=0.375,
     return spec
 
 
+def ext_spec(port_weight=None):
+    """python/test/test_ext_neuron.py:5-25: a spike generator (50, 100,
+    400, 600 ms) into ports 0-2 of an ext_neuron with weights 0.1 / 0.2 /
+    0.5 and delays 1 / 50 / 100 ms, plus one parrot neuron per delay as
+    the timing reference."""
+    ext = {"model": "ext_neuron", "n": 1, "n_port": 3}
+    if port_weight is not None:
+        ext["params"] = {"port_weight": port_weight}
+    spec = {"groups": [ext,
+                       {"model": "parrot_neuron", "n": 3,
+                        "params": {"hold_spike_height": 1.0}},
+                       {"model": "spike_generator", "n": 1}],
+            "spike_gen": [{"node": 4, "times": [50.0, 100.0, 400.0, 600.0]}],
+            "conns": [], "records": []}
+    for p, (w, d) in enumerate(zip([0.1, 0.2, 0.5], [1.0, 50.0, 100.0])):
+        spec["conns"].append({"src": [4], "tgt": [0], "weight": [w],
+                              "delay": [d], "port": p})
+        spec["conns"].append({"src": [4], "tgt": [1 + p], "weight": [w],
+                              "delay": [d], "port": 0})
+        spec["records"].append((0, "port_value", p))
+    for p in range(3):
+        spec["records"].append((1 + p, "V", 0))
+    return spec
+
+
 def load_stdp_cases():
     import json
     import os

@@ -588,6 +588,41 @@ def test_point_model_network_bit_exact(ngpu, model):
     assert eng == orc
 
 
+def test_ext_neuron_bit_exact(ngpu):
+    """ext_neuron (src/ext_neuron.cu:23-39) fed by a spike generator
+    (python/test/test_ext_neuron.py's synapses) and a 5 kHz Poisson
+    generator on ports 1 and 2: every port_value equals the FP32 oracle's,
+    and GetExtNeuronInputSpikes (src/neurongpu.cu:1540-1573) returns the
+    non-zero ports of the current step."""
+    log("case ext_neuron")
+    spec = nb.ext_spec([1.0, 2.0, 0.5])
+    spec["groups"].append({"model": "poisson_generator", "n": 1})
+    spec["poisson"] = [{"node": 5, "rate": 5000.0}]
+    for p, w in ((1, 0.25), (2, 0.5)):
+        spec["conns"].append({"src": [5], "tgt": [0], "weight": [w],
+                              "delay": [0.3], "port": p})
+    _, recs = nb.build_engine(ngpu, spec)
+    n_ext, _, _ = ngpu.GetExtNeuronInputSpikes(include_zeros=True)
+    assert n_ext == [0, 0, 0]
+    ngpu.Simulate(51.2)
+    nodes, ports, hs = ngpu.GetExtNeuronInputSpikes()
+    ngpu.Simulate(748.8)
+    data = np.column_stack([ngpu.GetRecordDataArray(r)[:, 1] for r in recs])
+    # the oracle in the same two calls: spike_generator indexes spikes by
+    # the step within the Simulate call (src/neurongpu.cu:482)
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(512)
+    net.run(7488)
+    orc = net.records()[:, 1:].astype(np.float32)
+    assert orc.shape == data.shape
+    assert np.array_equal(orc, data)
+    assert np.count_nonzero(data[:, 1]) > 1000     # the Poisson input
+    row = 512                                       # after Simulate(51.2)
+    want = [(0, p, float(orc[row, p])) for p in range(3) if orc[row, p] != 0]
+    assert list(zip(nodes, ports, hs)) == want
+    assert (0, 0, float(np.float32(0.1))) in want
+
+
 @pytest.mark.parametrize("launch", [{"delivery_mode": 0},
                                     {"delivery_mode": 1},
                                     {"graph_steps": 16}])

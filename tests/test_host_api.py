@@ -295,6 +295,19 @@ def test_point_model_defaults(ngpu, model, defaults, group):
             ngpu.SetNeuronGroupParam(ngpu.NodeSeq(ns.i0, 2), group[0], 1.0)
 
 
+def test_ext_neuron_host(ngpu):
+    """ext_neuron: two port variables and one port parameter per port
+    (src/ext_neuron.h:46-60); GetExtNeuronInputSpikes lists every port
+    with include_zeros."""
+    ns = ngpu.Create("ext_neuron", 2, 3)
+    assert ngpu.GetPortVarNames(ns[0]) == ["port_input", "port_value"]
+    assert ngpu.GetStatus(ns[1], "port_weight") == [1.0, 1.0, 1.0]
+    nodes, ports, hs = ngpu.GetExtNeuronInputSpikes(include_zeros=True)
+    assert nodes == [0, 0, 0, 1, 1, 1] and ports == [0, 1, 2] * 2
+    assert hs == [0.0] * 6
+    assert ngpu.GetExtNeuronInputSpikes() == ([], [], [])
+
+
 def test_create_poisson_generator(ngpu):
     """src/neurongpu.cu:245-265: one group, created before calibration,
     with no receptor ports; a second call or n_node <= 0 is an error."""

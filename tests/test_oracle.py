@@ -550,6 +550,31 @@ def test_delivery_delay_rows():
         assert rows == set(i + off for i in src_rows if i + off < n_rows)
 
 
+def test_ext_neuron_port_values():
+    """ext_neuron (src/ext_neuron.cu:23-39) shows each input in port_value
+    for exactly one step, at the step a parrot neuron with the same
+    synapse shows it (the delivery timing pinned above), scaled by
+    port_weight."""
+    net, _ = nb.build_oracle(nb.ext_spec(), "f32")
+    net.run(8000)
+    r = net.records()
+    for p in range(3):
+        assert np.array_equal(r[:, 1 + p], r[:, 4 + p])
+        assert np.count_nonzero(r[:, 1 + p]) == 4
+    # the generator's first spike, at record row i, arrives through the
+    # 1 / 50 / 100 ms synapses at rows i + 11 / 501 / 1001
+    st, nd, _ = net.spikes()
+    i = int(st[nd == 4][0]) + 1
+    for p, (off, w) in enumerate([(11, 0.1), (501, 0.2), (1001, 0.5)]):
+        assert r[i + off, 1 + p] == np.float32(w)
+    net, _ = nb.build_oracle(nb.ext_spec([1.0, 2.0, 0.5]), "f32")
+    net.run(8000)
+    r2 = net.records()
+    for p, pw in enumerate([1.0, 2.0, 0.5]):
+        assert np.array_equal(r2[:, 1 + p],
+                              (r[:, 1 + p] * np.float32(pw)).astype(np.float32))
+
+
 # ---------------------------------------------------------------------
 # network statistics (python/test/test_brunel_list.py:11,132-148)
 # ---------------------------------------------------------------------

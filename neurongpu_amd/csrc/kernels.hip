@@ -59,7 +59,9 @@ __device__ __forceinline__ void aeif_rhs(const AeifLocal<P> &p, float refr,
   t = t + I_syn;
   t = t - y[1];
   t = t + p.I_e;
-  dy[0] = rf ? 0.0f : t * p.inv_C_m;
+  // refractory: dV = 0, by a 0/1 factor rather than a branch, so that a
+  // wave never splits on the refractory state
+  dy[0] = (t * p.inv_C_m) * (rf ? 0.0f : 1.0f);
   float u = p.a * (V - p.E_L);
   u = u - y[1];
   dy[1] = u * p.inv_tau_w;

@@ -16,12 +16,15 @@ namespace ngpu {
 // exp(x): Cody-Waite reduction x = k ln2 + r (ln2 split 0x3F317200 +
 // 1.42860677e-06), exp(r) by the degree-7 Taylor polynomial on
 // |r| <= ln2/2, then 2^k applied as two exact power-of-two products.
+// Written without branches or selects: x is clamped to [-104, 88.8] by
+// NaN-propagating minimum/maximum, and the polynomial path already gives
+// +inf at 88.8 and +0 at -104 (and NaN for NaN), the values beyond the
+// range, so a wave never splits on the argument's range.
 __device__ __forceinline__ float dev_expf(float x) {
-  if (x != x) return x;
-  if (x > 88.8f) return __uint_as_float(0x7F800000u);
-  if (x < -104.0f) return 0.0f;
-  float kf = rintf(x * 1.44269502f);
-  float r = __builtin_fmaf(kf, -0.693145751953125f, x);
+  const float xc = __builtin_elementwise_minimum(
+      __builtin_elementwise_maximum(x, -104.0f), 88.8f);
+  float kf = rintf(xc * 1.44269502f);
+  float r = __builtin_fmaf(kf, -0.693145751953125f, xc);
   r = __builtin_fmaf(kf, -1.42860677e-06f, r);
   float p = 1.98412701e-04f;
   p = __builtin_fmaf(p, r, 1.38888892e-03f);
@@ -40,18 +43,15 @@ __device__ __forceinline__ float dev_expf(float x) {
 }
 
 // log(x) for x > 0: x = 2^e m with m in [sqrt(1/2), sqrt(2)),
-// log m = 2 atanh((m-1)/(m+1)) by the odd series to s^9.
+// log m = 2 atanh((m-1)/(m+1)) by the odd series to s^9.  Branch-free like
+// dev_expf: x <= 0, +inf and NaN are selected at the end.
 __device__ __forceinline__ float dev_logf(float x) {
-  if (x != x) return x;
-  if (x <= 0.0f) return __uint_as_float(0xFF800000u);
-  if (x == __uint_as_float(0x7F800000u)) return x;
   uint32_t b = __float_as_uint(x);
   int e = (int)((b >> 23) & 0xFFu) - 127;
   float m = __uint_as_float((b & 0x007FFFFFu) | 0x3F800000u);
-  if (m > 1.41421354f) {
-    m = m * 0.5f;
-    e = e + 1;
-  }
+  const bool hi = m > 1.41421354f;
+  m = hi ? m * 0.5f : m;
+  e = hi ? e + 1 : e;
   float s = (m - 1.0f) / (m + 1.0f);
   float s2 = s * s;
   float q = 1.11111112e-01f;
@@ -61,8 +61,11 @@ __device__ __forceinline__ float dev_logf(float x) {
   float two_s = 2.0f * s;
   float lm = __builtin_fmaf(two_s * s2, q, two_s);
   float ef = (float)e;
-  return __builtin_fmaf(ef, 0.693145751953125f,
-                        __builtin_fmaf(ef, 1.42860677e-06f, lm));
+  float res = __builtin_fmaf(ef, 0.693145751953125f,
+                             __builtin_fmaf(ef, 1.42860677e-06f, lm));
+  res = x == __uint_as_float(0x7F800000u) ? x : res;
+  res = x <= 0.0f ? __uint_as_float(0xFF800000u) : res;
+  return x != x ? x : res;
 }
 
 // err^(-1/4) and err^(-1/5) of the step-size controller.

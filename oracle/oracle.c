@@ -394,7 +394,8 @@ static void aeif_rhs(int nvar, const REAL *y, REAL *dy, const void *ctx) {
   t = t + I_syn;
   t = t - y[1];
   t = t + p->I_e;
-  dy[0] = refr ? (REAL)0 : t * p->inv_C_m;
+  /* refractory: dV = t / C_m times 0 (the kernels' form) */
+  dy[0] = (t * p->inv_C_m) * (refr ? (REAL)0 : (REAL)1);
   REAL u = p->a * (V - p->E_L);
   u = u - y[1];
   dy[1] = u * p->inv_tau_w;

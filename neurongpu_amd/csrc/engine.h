@@ -143,7 +143,9 @@ struct Group {
   DBuf<int> d_spike_count, d_n_rec_times;
   DBuf<float> d_rec_times;
   DBuf<unsigned char> d_strag;      // resume records of the two-pass update
-  DBuf<unsigned int> d_strag_n;     // [2][64] sub-list lengths
+  DBuf<unsigned int> d_strag_n;     // [4][2][64] sub-list lengths
+  DBuf<unsigned long long> d_mask;  // [2][n_waves] second-pass lanes
+  DBuf<float> d_carry;              // [2][n_port][n] input of skipped neurons
   // synchronisation between host mirror and device
   bool host_newer = true;   // host arrays changed since last upload
   bool dev_newer = false;   // device advanced since last download
@@ -266,6 +268,19 @@ class Engine {
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void EnsureStream2();
+  // k > 0: while delivery is binned (and nothing needs the end-of-step
+  // state of every neuron), the second RK5 pass of a step runs on the
+  // second stream beside the next step's first pass and this step's
+  // delivery.  The next first pass leaves that pass's neurons to the next
+  // second pass, for at most k steps in a row; then a first pass waits
+  // for the second pass before it and takes every neuron again.
+  // 0: the second pass ends each step.
+  int step_overlap = 2;
+  bool ov_async_prev = false;   // the last step's second pass may still run
+  bool ov_skip = false;         // this step's first pass skips its neurons
+  int ov_run = 0;               // steps in a row with ov_skip
+  hipEvent_t ev_p1 = nullptr, ev_p2[2] = {nullptr, nullptr};
+  int ev_p2_last = 0;
   unsigned long long *h_ev_snap = nullptr;  // pinned copy of the counters
   hipEvent_t ev_snap = nullptr;
   bool snap_pending = false;

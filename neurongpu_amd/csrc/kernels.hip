@@ -504,11 +504,17 @@ void aeif_pass1_kernel(AeifArgs A) {
   const StepNow sn = step_now(A.out, &A.fold, A.clock);
   const double x1 = A.clock ? sn.t : A.x1;
   const long long gs = aeif_gs(A);
-  unsigned int *strag_n = A.strag_n2 + kStragLists * (gs & 1);
-  unsigned int *strag_n_next = A.strag_n2 + kStragLists * ((gs + 1) & 1);
-  __shared__ uint32_t s_n, s_base;
-  if (threadIdx.x == 0) s_n = 0u;
-  if (blockIdx.x == 0 && threadIdx.x < kStragLists)
+  const int sel = (int)(gs & 3);
+  unsigned int *strag_n = A.strag_n2 + 2 * kStragLists * sel;
+  unsigned int *strag_n_next = A.strag_n2 + 2 * kStragLists * ((gs + 1) & 3);
+  StragRec *strag = reinterpret_cast<StragRec *>(A.strag) +
+                    (size_t)sel * 2 * kStragLists * A.strag_cap;
+  __shared__ uint32_t s_n, s_base, s_nc, s_basec;
+  if (threadIdx.x == 0) {
+    s_n = 0u;
+    s_nc = 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2 * kStragLists)
     strag_n_next[threadIdx.x] = 0u;
   __syncthreads();
   uint32_t slot = 0xFFFFFFFFu;
@@ -518,8 +524,23 @@ void aeif_pass1_kernel(AeifArgs A) {
   int ii = active ? i : 0;
   int node = A.i0 + ii;
   int nsp = 0, n_acc = 0, n_rej = 0;
-  bool finished = false;
-  if (active) {
+  bool finished = false, skipped = false;
+  if (active && A.skip_prev) {
+    const unsigned long long m =
+        A.mask[(size_t)((gs - 1) & 1) * A.n_waves + (i >> 6)];
+    skipped = (m >> (i & 63)) & 1ull;
+  }
+  uint32_t cslot = 0xFFFFFFFFu;
+  if (skipped) {
+    // owned by the previous step's second pass, which may still run:
+    // fold only the input, the second pass of this step runs the step
+    float in[P];
+    gather_input<P>(A.fold, sn, node, in);
+    float *carry = A.carry + (size_t)(gs & 1) * P * A.n;
+#pragma unroll
+    for (int q = 0; q < P; q++) carry[(size_t)q * A.n + i] = in[q];
+    cslot = atomicAdd(&s_nc, 1u);
+  } else if (active) {
     AeifLocal<P> p;
     aeif_load_params<P, UNI>(A, i, p);
     float y[N], k1[N], inv_scal[N];
@@ -568,16 +589,34 @@ void aeif_pass1_kernel(AeifArgs A) {
   }
   emit_spikes(A.out, sn, active && finished, ii, node, nsp, (float)nsp);
   aeif_count<P>(A, n_acc, n_rej);
+  if (A.mask) {
+    const unsigned long long m =
+        __ballot(slot != 0xFFFFFFFFu || cslot != 0xFFFFFFFFu);
+    if (lane_id() == 0 && (unsigned long long)(i >> 6) < A.n_waves)
+      A.mask[(size_t)(gs & 1) * A.n_waves + (i >> 6)] = m;
+  }
   // this block's resume records go to sub-list blockIdx % kStragLists,
-  // at an offset reserved with one atomic per block
+  // at an offset reserved with one atomic per block (and its catch-up
+  // entries to the same sub-list of the catch-up lists)
   __syncthreads();
   const uint32_t list = blockIdx.x % kStragLists;
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     s_base = s_n ? atomicAdd(strag_n + list, s_n) : 0u;
+    s_basec = s_nc ? atomicAdd(strag_n + kStragLists + list, s_nc) : 0u;
+  }
   __syncthreads();
   if (slot != 0xFFFFFFFFu)
-    reinterpret_cast<StragRec *>(A.strag)[(size_t)list * A.strag_cap +
-                                          s_base + slot] = rec;
+    strag[(size_t)list * A.strag_cap + s_base + slot] = rec;
+  if (cslot != 0xFFFFFFFFu) {
+    StragRec c;
+    c.i = (uint32_t)i;
+    c.nsp = 0u;
+    c.n_att = 0u;
+    c.flags = 1u;
+    c.h_begin = 0.0f;
+    c.err_last = 1.0f;
+    strag[(size_t)(kStragLists + list) * A.strag_cap + s_basec + cslot] = c;
+  }
 }
 
 // Second pass: the listed neurons, run to the end of the step.  The
@@ -593,11 +632,17 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
                                                 uint32_t vblock,
                                                 uint32_t vgrid) {
   constexpr int N = 2 + 2 * P;
+  // the pass is a few long dependent chains: its waves issue first when
+  // they share a SIMD with the first pass or the delivery
+  __builtin_amdgcn_s_setprio(3);
   const StepNow sn = step_now(A.out, &A.fold, A.clock);
   const double x1 = A.clock ? sn.t : A.x1;
-  const unsigned int *strag_n =
-      A.strag_n2 + kStragLists * (aeif_gs(A) & 1);
-  __shared__ uint32_t s_pre[kStragLists + 1];
+  const long long gs = aeif_gs(A);
+  // this step's set: resume records, then the catch-up entries (neurons
+  // the first pass of this step skipped)
+  const unsigned int *strag_n = A.strag_n2 + 2 * kStragLists * (gs & 3);
+  const unsigned int *strag_n_prev = strag_n + kStragLists;
+  __shared__ uint32_t s_pre[kStragLists + 1], s_pre2[kStragLists + 1];
   // proposer weights, staged once per block (read at every step start)
   __shared__ __attribute__((aligned(16)))
   unsigned char s_pk[PROP ? kPropPkBytes : 16];
@@ -607,23 +652,30 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
     for (int k = threadIdx.x; k < kPropPkBytes / 16; k += kBlock)
       dst[k] = src[k];
   }
-  if (threadIdx.x < 64) {  // one wave: exclusive scan of the 64 lengths
-    uint32_t c = strag_n[threadIdx.x];
+  if (threadIdx.x < 128) {  // two waves: exclusive scans of 64 lengths
+    const int l = threadIdx.x & 63;
+    const bool second = threadIdx.x >= 64;
+    uint32_t c = second ? (A.skip_prev ? strag_n_prev[l] : 0u) : strag_n[l];
     uint32_t incl = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       uint32_t v = __shfl_up(incl, off);
-      if ((int)threadIdx.x >= off) incl += v;
+      if (l >= off) incl += v;
     }
-    s_pre[threadIdx.x] = incl - c;
-    if (threadIdx.x == 63) s_pre[kStragLists] = incl;
+    uint32_t *pre = second ? s_pre2 : s_pre;
+    pre[l] = incl - c;
+    if (l == 63) pre[kStragLists] = incl;
   }
   __syncthreads();
-  const uint32_t total = s_pre[kStragLists];
+  const uint32_t n_cont = s_pre[kStragLists];
+  const uint32_t total = n_cont + s_pre2[kStragLists];
   const uint32_t n_lanes = vgrid * kBlock;
   uint32_t j = vblock * kBlock + threadIdx.x;
   if (vblock * kBlock >= total) return;  // whole block beyond the list
-  const StragRec *list = reinterpret_cast<const StragRec *>(A.strag);
+  const StragRec *list = reinterpret_cast<const StragRec *>(A.strag) +
+                        (size_t)(gs & 3) * 2 * kStragLists * A.strag_cap;
+  const StragRec *list_prev = list + (size_t)kStragLists * A.strag_cap;
+  const float *carry = A.carry + (size_t)(gs & 1) * P * A.n;
   AeifLocal<P> p;
   float y[N], k1[N], inv_scal[N];
   double x = 0.0;
@@ -634,18 +686,24 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
   bool need_k1 = false, have = false;
   for (;;) {
     if (!have && j < total) {
-      // sub-list of entry j: last l with s_pre[l] <= j
+      // entries [0, n_cont) resume this step's deferred neurons, the rest
+      // are the catch-up neurons; sub-list of an entry: last l with
+      // pre[l] <= its index
+      const bool catch_up = j >= n_cont;
+      const uint32_t jj = catch_up ? j - n_cont : j;
+      const uint32_t *pre = catch_up ? s_pre2 : s_pre;
       int lo = 0, hi = kStragLists;
       while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
-        if (s_pre[mid] <= j) lo = mid; else hi = mid;
+        if (pre[mid] <= jj) lo = mid; else hi = mid;
       }
-      StragRec r = list[(size_t)lo * A.strag_cap + (j - s_pre[lo])];
+      StragRec r = (catch_up ? list_prev : list)[(size_t)lo * A.strag_cap +
+                                                  (jj - pre[lo])];
       j += n_lanes;
       i = (int)r.i;
-      nsp = (int)r.nsp;
-      n_att = (int)r.n_att;
-      err_last = r.err_last;
+      nsp = catch_up ? 0 : (int)r.nsp;
+      n_att = catch_up ? 0 : (int)r.n_att;
+      err_last = catch_up ? 1.0f : r.err_last;
       aeif_load_params<P, UNI>(A, i, p);
       y[0] = A.V[i];
       y[1] = A.w[i];
@@ -657,7 +715,15 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       x = A.x[i];
       h = A.h[i];
       refr = A.refr[i];
-      if (r.flags & 1u) {
+      if (catch_up) {
+        // the whole step, as the first pass runs it: input fold, then
+        // RK5 steps from the stored step size
+#pragma unroll
+        for (int q = 0; q < P; q++)
+          y[3 + 2 * q] =
+              __builtin_fmaf(carry[(size_t)q * A.n + i], p.g0[q], y[3 + 2 * q]);
+        need_k1 = true;
+      } else if (r.flags & 1u) {
         need_k1 = true;  // saved at a step boundary: a new RK5 step starts
       } else {
         aeif_rk5_begin<P>(p, refr, y, r.h_begin, k1, inv_scal);

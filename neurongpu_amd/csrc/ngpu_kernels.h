@@ -127,12 +127,25 @@ struct AeifArgs {
   int do_step;                // 0: fold only (end-of-simulation flush)
   // two-pass update: first pass with at most `budget` RK5 attempts per
   // neuron, the rest finished by a second pass over 64 sub-lists of
-  // resume records (strag: [64][strag_cap]); strag_n2 holds two sets of
-  // 64 sub-list lengths, used by the steps of even and odd gs in turn (the
-  // first pass of a step clears the other set for the next step);
-  // strag == nullptr selects the single-pass kernel
+  // resume records.  Four sets of lists serve the steps in turn by gs
+  // mod 4 (strag: [4][2][64][strag_cap] records, strag_n2: [4][2][64]
+  // lengths; per set, the deferred neurons' resume records, then the
+  // catch-up list below); the first pass of a step clears the lengths of
+  // the next step's set.  strag == nullptr selects the single-pass kernel
   void *strag;
   unsigned int *strag_n2;
+  // Steps overlapped with the previous step's second pass.  The first
+  // pass writes, per wave, the ballot of the lanes the second pass of its
+  // step owns (mask: [2][n_waves] by gs parity).  With skip_prev set it
+  // leaves the lanes the previous second pass owned alone (that pass may
+  // still run): it only folds their input into carry ([2][P][n] by gs
+  // parity) and lists them (catch-up); the second pass then runs their
+  // whole step, and they stay with the second passes until a step runs
+  // without skip_prev.
+  unsigned long long *mask;
+  unsigned long long n_waves;
+  float *carry;
+  int skip_prev;
   long long gs;
   const StepClock *clock;     // nullptr: step values from the arguments
   unsigned long long strag_cap;

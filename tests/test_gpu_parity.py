@@ -372,7 +372,11 @@ def test_ensemble_routing_matches_instances_alone(ngpu):
     {"delivery_grid": 256, "delivery_tile_max": 8, "fuse_delivery": 0},
     {"delivery_grid": 4096, "delivery_tile_max": 1},
     {"delivery_mode": 1},
-    {"delivery_mode": 1, "delivery_streams": 1},
+    {"delivery_mode": 1, "step_overlap": 1},
+    {"delivery_mode": 1, "step_overlap": 1024},
+    {"delivery_mode": 1, "rk5_budget": 1, "rk5_pass2_grid": 64},
+    {"delivery_mode": 1, "step_overlap": 0},
+    {"delivery_mode": 1, "step_overlap": 0, "delivery_streams": 1},
     {"delivery_mode": 1, "delivery_grid": 64, "delivery_tile_max": 4},
 ])
 def test_launch_shapes_do_not_change_results(ngpu, settings):
@@ -413,6 +417,44 @@ def test_graph_steps_match_eager(ngpu, settings):
     spec = nb.ensemble_spec(_ensemble_instances())
     ref = sorted(_run_spikes_calls(ngpu, spec, 4, 50.0, {"graph_steps": 0}))
     got = sorted(_run_spikes_calls(ngpu, spec, 4, 50.0, settings))
+    assert len(ref) > 100
+    assert got == ref
+
+
+def test_step_overlap_over_calls(ngpu):
+    """Binned delivery with each step's second RK5 pass overlapped with the
+    next step (the default), over four Simulate calls with host reads in
+    between: spikes identical to steps that end with their second pass."""
+    log("case step_overlap_calls")
+    spec = nb.ensemble_spec(_ensemble_instances())
+    ref = sorted(_run_spikes_calls(ngpu, spec, 4, 50.0, {"step_overlap": 0}))
+    got = sorted(_run_spikes_calls(ngpu, spec, 4, 50.0, {"delivery_mode": 1}))
+    assert len(ref) > 100
+    assert got == ref
+
+
+def test_step_overlap_transitions(ngpu):
+    """Overlapped steps (binned delivery) and steps that end with their
+    second pass (atomic delivery) alternate inside one run of the stepping
+    API, with a host read of the state in the middle: the first plain step
+    after an overlapped one finishes the neurons the overlapped step left
+    over, and the spikes equal those of plain steps throughout."""
+    log("case step_overlap_transitions")
+    spec = nb.ensemble_spec(_ensemble_instances())
+    ref = sorted(_run_spikes(ngpu, spec, 200.0, {"step_overlap": 0}))
+    groups, _ = nb.build_engine(ngpu, spec)
+    ngpu.ActivateRecSpikeTimes(groups[0], 2000)
+    ngpu.StartSimulation()
+    for k in range(2000):
+        if k % 400 == 0:
+            ngpu.SetKernelStatus("delivery_mode", 1)
+        elif k % 400 == 250:
+            ngpu.SetKernelStatus("delivery_mode", 0)
+        if k == 1111:
+            ngpu.GetStatus(groups[0][0], "V_m")
+        ngpu.SimulationStep()
+    ngpu.EndSimulation()
+    got = sorted(nb.engine_spikes(ngpu, groups[0]))
     assert len(ref) > 100
     assert got == ref
 

@@ -413,7 +413,7 @@ def test_launch_settings(ngpu):
                 "delivery_tile_max": 64, "delivery_mode": 2,
                 "delivery_bin_capacity": 1 << 27,
                 "delivery_auto_events": 500000, "graph_steps": 0,
-                "delivery_streams": 2}
+                "delivery_streams": 2, "step_overlap": 2}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)
@@ -425,6 +425,7 @@ def test_launch_settings(ngpu):
     for k, bad in (("delivery_mode", 3), ("delivery_tile_max", 3),
                    ("delivery_grid", 0), ("rk5_budget", 0),
                    ("delivery_bin_capacity", 0), ("graph_steps", -1),
-                   ("graph_steps", 7), ("delivery_streams", 3)):
+                   ("graph_steps", 7), ("delivery_streams", 3),
+                   ("step_overlap", -1), ("step_overlap", 1025)):
         with pytest.raises(ValueError):
             ngpu.SetKernelStatus(k, bad)

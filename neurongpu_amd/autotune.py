@@ -6,7 +6,8 @@ stretches of the running simulation.  The settings are:
 * the delivery mode (one float atomic per synaptic event, or events
   binned by accumulator range and summed in LDS first), the delivery grid
   and the largest number of (spike, delay) pairs per wave tile;
-* whether the delivery shares the second RK5 pass's launch;
+* whether the delivery shares the second RK5 pass's launch, and how many
+  steps in a row the second pass may overlap the next step;
 * whether Simulate replays captured hipGraphs of 16 steps or launches
   every step from the host.
 
@@ -24,6 +25,7 @@ CANDIDATES = {
     "delivery_grid": [1024, 2048, 4096],
     "delivery_tile_max": [8, 16, 64],
     "fuse_delivery": [1, 0],
+    "step_overlap": [2, 0, 1, 4],
     "graph_steps": [0, 16],
 }
 

@@ -278,6 +278,7 @@ class Engine {
   int step_overlap = 2;
   bool ov_async_prev = false;   // the last step's second pass may still run
   bool ov_skip = false;         // this step's first pass skips its neurons
+  bool ov_mask = false;         // this step's second pass may overlap
   int ov_run = 0;               // steps in a row with ov_skip
   hipEvent_t ev_p1 = nullptr, ev_p2[2] = {nullptr, nullptr};
   int ev_p2_last = 0;

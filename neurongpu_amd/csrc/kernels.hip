@@ -497,7 +497,7 @@ __device__ __forceinline__ long long aeif_gs(const AeifArgs &A) {
 // fits in 80 VGPRs, which allows 6 wavefronts per SIMD instead of 5 (7 %
 // faster on c1, measured); for more ports or per-neuron parameters the same
 // limit would spill to scratch.
-template <int P, bool UNI, int W>
+template <int P, bool UNI, int W, bool OV>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 void aeif_pass1_kernel(AeifArgs A) {
   constexpr int N = 2 + 2 * P;
@@ -525,7 +525,7 @@ void aeif_pass1_kernel(AeifArgs A) {
   int node = A.i0 + ii;
   int nsp = 0, n_acc = 0, n_rej = 0;
   bool finished = false, skipped = false;
-  if (active && A.skip_prev) {
+  if (OV && active && A.skip_prev) {
     const unsigned long long m =
         A.mask[(size_t)((gs - 1) & 1) * A.n_waves + (i >> 6)];
     skipped = (m >> (i & 63)) & 1ull;
@@ -589,7 +589,7 @@ void aeif_pass1_kernel(AeifArgs A) {
   }
   emit_spikes(A.out, sn, active && finished, ii, node, nsp, (float)nsp);
   aeif_count<P>(A, n_acc, n_rej);
-  if (A.mask) {
+  if (OV && A.mask_write) {
     const unsigned long long m =
         __ballot(slot != 0xFFFFFFFFu || cslot != 0xFFFFFFFFu);
     if (lane_id() == 0 && (unsigned long long)(i >> 6) < A.n_waves)
@@ -602,12 +602,12 @@ void aeif_pass1_kernel(AeifArgs A) {
   const uint32_t list = blockIdx.x % kStragLists;
   if (threadIdx.x == 0) {
     s_base = s_n ? atomicAdd(strag_n + list, s_n) : 0u;
-    s_basec = s_nc ? atomicAdd(strag_n + kStragLists + list, s_nc) : 0u;
+    if (OV) s_basec = s_nc ? atomicAdd(strag_n + kStragLists + list, s_nc) : 0u;
   }
   __syncthreads();
   if (slot != 0xFFFFFFFFu)
     strag[(size_t)list * A.strag_cap + s_base + slot] = rec;
-  if (cslot != 0xFFFFFFFFu) {
+  if (OV && cslot != 0xFFFFFFFFu) {
     StragRec c;
     c.i = (uint32_t)i;
     c.nsp = 0u;
@@ -627,14 +627,14 @@ void aeif_pass1_kernel(AeifArgs A) {
 // end exit at once.
 // PROP: the learned proposer's inference is compiled in (its weights in
 // LDS, the MFMA registers); without it the pass keeps its occupancy.
-template <int P, bool UNI, bool PROP>
+template <int P, bool UNI, bool PROP, bool CU>
 __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
                                                 uint32_t vblock,
                                                 uint32_t vgrid) {
   constexpr int N = 2 + 2 * P;
   // the pass is a few long dependent chains: its waves issue first when
   // they share a SIMD with the first pass or the delivery
-  __builtin_amdgcn_s_setprio(3);
+  if (CU) __builtin_amdgcn_s_setprio(3);
   const StepNow sn = step_now(A.out, &A.fold, A.clock);
   const double x1 = A.clock ? sn.t : A.x1;
   const long long gs = aeif_gs(A);
@@ -652,7 +652,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
     for (int k = threadIdx.x; k < kPropPkBytes / 16; k += kBlock)
       dst[k] = src[k];
   }
-  if (threadIdx.x < 128) {  // two waves: exclusive scans of 64 lengths
+  if (threadIdx.x < (CU ? 128 : 64)) {  // exclusive scans of 64 lengths
     const int l = threadIdx.x & 63;
     const bool second = threadIdx.x >= 64;
     uint32_t c = second ? (A.skip_prev ? strag_n_prev[l] : 0u) : strag_n[l];
@@ -668,7 +668,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
   }
   __syncthreads();
   const uint32_t n_cont = s_pre[kStragLists];
-  const uint32_t total = n_cont + s_pre2[kStragLists];
+  const uint32_t total = n_cont + (CU ? s_pre2[kStragLists] : 0u);
   const uint32_t n_lanes = vgrid * kBlock;
   uint32_t j = vblock * kBlock + threadIdx.x;
   if (vblock * kBlock >= total) return;  // whole block beyond the list
@@ -689,7 +689,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       // entries [0, n_cont) resume this step's deferred neurons, the rest
       // are the catch-up neurons; sub-list of an entry: last l with
       // pre[l] <= its index
-      const bool catch_up = j >= n_cont;
+      const bool catch_up = CU && j >= n_cont;
       const uint32_t jj = catch_up ? j - n_cont : j;
       const uint32_t *pre = catch_up ? s_pre2 : s_pre;
       int lo = 0, hi = kStragLists;
@@ -797,9 +797,9 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
   aeif_count<P>(A, n_acc, n_rej);
 }
 
-template <int P, bool UNI, bool PROP>
+template <int P, bool UNI, bool PROP, bool CU>
 __global__ __launch_bounds__(kBlock) void aeif_pass2_kernel(AeifArgs A) {
-  aeif_pass2_body<P, UNI, PROP>(A, blockIdx.x, gridDim.x);
+  aeif_pass2_body<P, UNI, PROP, CU>(A, blockIdx.x, gridDim.x);
 }
 
 template <bool PLASTIC>
@@ -846,13 +846,13 @@ __device__ __forceinline__ RevArgs clocked(const RevArgs &R) {
 // spikes of earlier steps and writes only the input accumulators), so the
 // latency-bound second pass and the atomic-bound delivery share the chip
 // without a second stream.
-template <int P, bool UNI, bool PROP>
+template <int P, bool UNI, bool PROP, bool CU>
 __global__ __launch_bounds__(kBlock) void aeif_tail_kernel(AeifArgs A,
                                                            DeliveryArgs D,
                                                            uint32_t n2) {
   clock_step(D.clock, D.clock_next);
   if (blockIdx.x < n2)
-    aeif_pass2_body<P, UNI, PROP>(A, blockIdx.x, n2);
+    aeif_pass2_body<P, UNI, PROP, CU>(A, blockIdx.x, n2);
   else
     delivery_body<false>(clocked(D), blockIdx.x - n2, gridDim.x - n2);
 }
@@ -1606,26 +1606,39 @@ void launch_aeif_p(const AeifArgs &a, hipStream_t s) {
   int grid = (a.n + kBlock - 1) / kBlock;
   if (grid <= 0) return;
   if (a.do_step && a.strag) {
+    // the overlap bookkeeping is compiled in only where it is used (groups
+    // of up to kOverlapMaxPort ports)
+    if constexpr (P <= kOverlapMaxPort) {
+      if (a.skip_prev || a.mask_write) {
+        if (a.uniform)
+          hipLaunchKernelGGL((aeif_pass1_kernel<P, true, (P <= 2 ? 6 : 1), true>),
+                             dim3(grid), dim3(kBlock), 0, s, a);
+        else
+          hipLaunchKernelGGL((aeif_pass1_kernel<P, false, 1, true>), dim3(grid),
+                             dim3(kBlock), 0, s, a);
+        return;
+      }
+    }
     if (a.uniform)
-      hipLaunchKernelGGL((aeif_pass1_kernel<P, true, (P <= 2 ? 6 : 1)>),
+      hipLaunchKernelGGL((aeif_pass1_kernel<P, true, (P <= 2 ? 6 : 1), false>),
                          dim3(grid), dim3(kBlock), 0, s, a);
     else
-      hipLaunchKernelGGL((aeif_pass1_kernel<P, false, 1>), dim3(grid),
+      hipLaunchKernelGGL((aeif_pass1_kernel<P, false, 1, false>), dim3(grid),
                          dim3(kBlock), 0, s, a);
   } else {
     hipLaunchKernelGGL(aeif_kernel<P>, dim3(grid), dim3(kBlock), 0, s, a);
   }
 }
 
-template <int P, bool UNI, bool PROP>
+template <int P, bool UNI, bool PROP, bool CU>
 void launch_pass2_v(const AeifArgs &a, const DeliveryArgs *d, int grid2,
                     int d_grid, hipStream_t s) {
   if (d)
-    hipLaunchKernelGGL((aeif_tail_kernel<P, UNI, PROP>), dim3(grid2 + d_grid),
+    hipLaunchKernelGGL((aeif_tail_kernel<P, UNI, PROP, CU>), dim3(grid2 + d_grid),
                        dim3(kBlock), delivery_lds_bytes(d->D, 0), s, a, *d,
                        (uint32_t)grid2);
   else
-    hipLaunchKernelGGL((aeif_pass2_kernel<P, UNI, PROP>), dim3(grid2),
+    hipLaunchKernelGGL((aeif_pass2_kernel<P, UNI, PROP, CU>), dim3(grid2),
                        dim3(kBlock), 0, s, a);
 }
 
@@ -1633,18 +1646,28 @@ template <int P>
 void launch_aeif_pass2_p(const AeifArgs &a, const DeliveryArgs *d,
                          int d_grid, hipStream_t s) {
   int grid2 = a.pass2_grid > 0 ? a.pass2_grid : 256;
+  // the catch-up of neurons an overlapped step left (without the proposer)
+  if constexpr (P <= kOverlapMaxPort) {
+    if (a.skip_prev || a.mask_write) {
+      if (a.uniform)
+        launch_pass2_v<P, true, false, true>(a, d, grid2, d_grid, s);
+      else
+        launch_pass2_v<P, false, false, true>(a, d, grid2, d_grid, s);
+      return;
+    }
+  }
   // the proposer's inference exists for groups of up to kPropMaxPort ports
   constexpr bool kProp = P <= kPropMaxPort;
   if (a.prop_pk && kProp) {
     if (a.uniform)
-      launch_pass2_v<P, true, kProp>(a, d, grid2, d_grid, s);
+      launch_pass2_v<P, true, kProp, false>(a, d, grid2, d_grid, s);
     else
-      launch_pass2_v<P, false, kProp>(a, d, grid2, d_grid, s);
+      launch_pass2_v<P, false, kProp, false>(a, d, grid2, d_grid, s);
   } else {
     if (a.uniform)
-      launch_pass2_v<P, true, false>(a, d, grid2, d_grid, s);
+      launch_pass2_v<P, true, false, false>(a, d, grid2, d_grid, s);
     else
-      launch_pass2_v<P, false, false>(a, d, grid2, d_grid, s);
+      launch_pass2_v<P, false, false, false>(a, d, grid2, d_grid, s);
   }
 }
 

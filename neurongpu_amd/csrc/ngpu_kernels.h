@@ -27,6 +27,9 @@ constexpr int kCounterSlots = 1024;   // spread statistic counters over L2
 constexpr uint32_t kPoissonTag = 0x504F4953u;
 constexpr uint32_t kLegacyPoissonTag = 0x4C504F49u;
 constexpr int kStragRecBytes = 24;    // resume record of the two-pass update
+// second RK5 passes overlap the next step for groups of up to this many
+// receptor ports
+constexpr int kOverlapMaxPort = 4;
 
 // Step clock in device memory.  Kernels launched with a non-null clock
 // read the step-dependent values from it instead of from their arguments,
@@ -146,6 +149,7 @@ struct AeifArgs {
   unsigned long long n_waves;
   float *carry;
   int skip_prev;
+  int mask_write;             // this step's second pass overlaps the next
   long long gs;
   const StepClock *clock;     // nullptr: step values from the arguments
   unsigned long long strag_cap;

@@ -88,6 +88,9 @@ struct ModelDesc {
 
 const ModelDesc &Model(ModelId id);
 int FindModel(const std::string &name);
+struct Engine;
+// generates a rule-only chunk on the host (api.cpp)
+void MaterializeRule(Engine &e, size_t chunk);
 // iaf_psc_exp propagator P32 (engine.cpp)
 double IafP32(double tau_syn, double tau, double C, double h);
 // inversion thresholds of Poisson(lambda) (engine.cpp); returns the table
@@ -208,6 +211,19 @@ class Engine {
   std::vector<int> node_group;
   int n_node = 0;
 
+  // A connection rule left for generation on the device at calibration
+  // (connect_rules.hip): node lists and per-connection arrays are copies
+  // of the caller's.  Queries made before calibration generate it on the
+  // host instead (MaterializeRule), with the same result.
+  struct ConnRule {
+    int rule = 0;                 // ConnGenRule
+    std::vector<int> s_arr, t_arr;
+    int s0 = 0, ns = 0, t0 = 0, nt = 0, K = 0;
+    int port = 0, sg = 0;
+    float w0 = 0.0f, d0 = 0.0f;
+    std::vector<float> w_arr, d_arr;
+    unsigned long long seed = 0, call = 0;
+  };
   // connections in coordinate form and insertion order, one chunk per
   // Connect call, kept until calibration
   struct CooChunk {
@@ -216,7 +232,10 @@ class Engine {
     std::unique_ptr<float[]> w;
     std::unique_ptr<uint16_t[]> d;        // delay index d_int
     std::unique_ptr<uint8_t[]> sg;
+    std::shared_ptr<ConnRule> rule;       // set: not generated yet
   };
+  // 1: Connect leaves the rules above to the device when it can
+  int device_connect = 1;
   std::vector<CooChunk> coo;
   size_t n_conn = 0;
   int coo_D = 0;                      // max delay index + 1

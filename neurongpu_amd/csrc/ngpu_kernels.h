@@ -31,7 +31,37 @@ constexpr int kStragRecBytes = 24;    // resume record of the two-pass update
 // receptor ports
 constexpr int kOverlapMaxPort = 4;
 
-// Step clock in device memory.  Kernels launched with a non-null clock
+// Connection rules generated on the device (connect_rules.hip).
+enum ConnGenRule {
+  kGenOneToOne = 0,
+  kGenAllToAll = 1,
+  kGenFixedTotalNumber = 2,
+  kGenFixedIndegree = 3,
+  kGenFixedOutdegree = 4,
+};
+constexpr size_t kConnGenMaxLds = 64 * 1024;  // sampler state per item
+struct ConnGenArgs {
+  int rule;
+  const int *s_arr, *t_arr;     // node lists (nullptr: ranges s0.., t0..)
+  int s0, ns, t0, nt;
+  int K;                        // in/outdegree
+  int cap;                      // rejection table size (set by the launcher)
+  uint32_t port_word;           // port << kPortShift
+  uint8_t sg_val;
+  float w0, d0, dt;
+  const float *w_arr, *d_arr;   // per connection, or nullptr
+  uint32_t k0, k1, call;        // stream key and call index
+  unsigned long long m;         // connections
+  uint32_t *src, *tp;           // outputs at the chunk's offset
+  float *w;
+  uint16_t *d;
+  uint8_t *sg;                  // nullptr without plasticity
+};
+// LDS bytes of the sampler of an in/outdegree rule (0 for the others)
+size_t ConnGenLdsBytes(int rule, int ns, int nt, int K, int *cap_out);
+const char *LaunchConnGen(const ConnGenArgs &a, void *stream);
+
+// Step clock in device memory.// Step clock in device memory.  Kernels launched with a non-null clock
 // read the step-dependent values from it instead of from their arguments,
 // so captured steps (hipGraph) replay unchanged.  Two clocks alternate
 // (even and odd steps of a capture): the last kernel of a step writes the

@@ -459,6 +459,74 @@ def test_step_overlap_transitions(ngpu):
     assert got == ref
 
 
+def _conn_table(ngpu, n):
+    ids = ngpu.GetConnections(ngpu.NodeSeq(0, n), ngpu.NodeSeq(0, n))
+    out = []
+    for c in ids:
+        st = ngpu.GetConnectionStatus(c)
+        out.append((st["source"], st["target"], st["port"], st["syn"],
+                    st["delay"], st["weight"]))
+    return out
+
+
+def _build_rules(ngpu, device):
+    """Every rule the device generates, with ranges and node lists, scalar
+    and per-connection weights and delays, both sampler paths (n >= 5k:
+    rejection; n < 5k: partial Fisher-Yates), a plastic synapse group, and
+    an indegree whose sampler does not fit the LDS (left to the host)."""
+    rng = np.random.default_rng(5)
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetRandomSeed(777)
+    ngpu.SetKernelStatus("device_connect", device)
+    a = ngpu.Create("aeif_cond_beta", 1200, 2)
+    b = ngpu.Create("iaf_psc_exp", 900)
+    big = ngpu.Create("parrot_neuron", 20000)
+    ngpu.CreateSynGroup("stdp", {})
+    n_tot = 1200 + 900 + 20000
+    m1 = 900 * 50
+    ngpu.Connect(a, b, {"rule": "fixed_indegree", "indegree": 50},
+                 {"weight": 0.5, "delay": {"array": rng.integers(1, 30, m1)
+                                           .astype(np.float32) * 0.1}})
+    ngpu.Connect(a[0:100], a, {"rule": "fixed_indegree", "indegree": 40},
+                 {"weight": 0.25, "delay": 1.0, "receptor": 1})
+    m3 = 500 * 30
+    ngpu.Connect(a[0:500], b, {"rule": "fixed_outdegree", "outdegree": 30},
+                 {"weight": {"array": rng.normal(1.0, 0.1, m3)
+                             .astype(np.float32)}, "delay": 2.0})
+    ngpu.Connect(b[0:60], a[0:60], {"rule": "fixed_outdegree", "outdegree": 20},
+                 {"weight": 1.5, "delay": 0.5, "synapse_group": 1})
+    src = rng.integers(0, 1200, 300).astype(np.int32)
+    tgt = (1200 + rng.integers(0, 900, 300)).astype(np.int32)
+    ngpu.Connect(list(map(int, src)), list(map(int, tgt)),
+                 {"rule": "fixed_total_number", "total_num": 5000},
+                 {"weight": 0.1, "delay": 0.3})
+    ngpu.Connect(list(map(int, src[:30])), list(map(int, tgt[:40])),
+                 {"rule": "all_to_all"}, {"weight": 0.2, "delay": 0.4})
+    ngpu.Connect(list(map(int, src[:70])), list(map(int, tgt[:70])),
+                 {"rule": "one_to_one"}, {"weight": 0.3, "delay": 0.6})
+    ngpu.Connect(big[0:20000], b[0:2], {"rule": "fixed_indegree",
+                                       "indegree": 3500},
+                 {"weight": 0.05, "delay": 0.2})
+    ngpu.Simulate(1.0)
+    return n_tot
+
+
+def test_device_connect_matches_host(ngpu):
+    """Connection rules generated on the device at calibration give the
+    synapse store the host generator gives: every connection's source,
+    target, port, synapse group, delay and weight, in the same order."""
+    log("case device_connect")
+    n = _build_rules(ngpu, 0)
+    host = _conn_table(ngpu, n)
+    _build_rules(ngpu, 1)
+    dev = _conn_table(ngpu, n)
+    log("connections", len(host))
+    assert len(host) == 900 * 50 + 1200 * 40 + 500 * 30 + 60 * 20 + 5000 + \
+        30 * 40 + 70 + 2 * 3500
+    assert dev == host
+
+
 def test_graph_steps_stdp_network(ngpu):
     """Plastic network of the bit-exact STDP test (dyadic weights): parrot
     spikes and every plastic weight identical with and without captured

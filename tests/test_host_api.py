@@ -413,7 +413,8 @@ def test_launch_settings(ngpu):
                 "delivery_tile_max": 64, "delivery_mode": 2,
                 "delivery_bin_capacity": 1 << 27,
                 "delivery_auto_events": 500000, "graph_steps": 0,
-                "delivery_streams": 2, "step_overlap": 2}
+                "delivery_streams": 2, "step_overlap": 2,
+                "device_connect": 1}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)
@@ -426,6 +427,7 @@ def test_launch_settings(ngpu):
                    ("delivery_grid", 0), ("rk5_budget", 0),
                    ("delivery_bin_capacity", 0), ("graph_steps", -1),
                    ("graph_steps", 7), ("delivery_streams", 3),
-                   ("step_overlap", -1), ("step_overlap", 1025)):
+                   ("step_overlap", -1), ("step_overlap", 1025),
+                   ("device_connect", 2)):
         with pytest.raises(ValueError):
             ngpu.SetKernelStatus(k, bad)

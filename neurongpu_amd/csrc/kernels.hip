@@ -1513,21 +1513,21 @@ __global__ __launch_bounds__(kBlock) void bin_sum_kernel(
 __global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs A0) {
   clock_step(A0.clock, A0.clock_next);
   const RevArgs A = clocked(A0);
-  const uint32_t lane = lane_id();
-  const uint32_t wave = (uint32_t)(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);
-  const uint32_t n_waves = gridDim.x * (kBlock / 64);
+  // one block per spiking node: its incoming plastic synapses are
+  // independent (each has this node as its only target), so all 256 lanes
+  // walk the reverse index together
   for (int k = 0; k <= A.max_den; k++) {
     long long s = A.gs - k;
     if (s < 0) break;
     long long slot = s % A.R;
     uint32_t n = A.ring_n[slot];
-    for (uint32_t e = wave; e < n; e += n_waves) {
+    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
       uint32_t node = A.ring_src[slot * A.cap + e];
       if (A.den_idx[node] != k) continue;
       if (A.last_rev[node] != A.T) continue;
       unsigned long long r0 = A.rev_ptr[node], r1 = A.rev_ptr[node + 1];
       uint16_t ti = (uint16_t)(A.T & 0xffff);
-      for (unsigned long long r = r0 + lane; r < r1; r += 64) {
+      for (unsigned long long r = r0 + threadIdx.x; r < r1; r += kBlock) {
         uint32_t syn = A.rev_idx[r];
         int sgp = A.syn_group[syn];
         if (sgp <= 0) continue;

@@ -47,6 +47,8 @@ METRIC = ("wall s per bio s + synaptic events/s (1M AdEx, 1B syn); "
           "RK5 stages/step learned vs plain")
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# build times of the last network, for the workload objects
+BUILD = {}
 # algorithmic work per unit (RK5 aeif P=2, delivery)
 RK5_BYTES_PER_NEURON_STEP = 72.0   # 2*4*NVAR + 8 h + 8 refractory + 4*P
 RK5_FLOP_ACCEPTED = 512.0
@@ -98,7 +100,8 @@ def build_c1(ngpu, n=1000000, seed=1234, tol=1e-3):
     del d_i
     ngpu.Connect(pg, neuron, {"rule": "all_to_all"},
                  {"weight": 0.1, "delay": 0.1, "receptor": 0})
-    log("  connections in %.1f s" % (time.time() - t))
+    BUILD["connect_s"] = time.time() - t
+    log("  connections in %.1f s" % BUILD["connect_s"])
     return neuron, n * (k_e + k_i)
 
 
@@ -130,7 +133,8 @@ def build_readme(ngpu, n=1000000, seed=1234):
     del d_i
     ngpu.Connect(pg, neuron, {"rule": "all_to_all"},
                  {"weight": 0.37, "delay": 0.2, "receptor": 0})
-    log("  connections in %.1f s" % (time.time() - t))
+    BUILD["connect_s"] = time.time() - t
+    log("  connections in %.1f s" % BUILD["connect_s"])
     return neuron, (n_e + n_i) * (ce + ci)
 
 
@@ -174,7 +178,8 @@ def build_c2(ngpu, scale=1.0, seed=2014):
             n_syn += k
         ngpu.Connect(pg[j:j + 1], pops[j], {"rule": "all_to_all"},
                      {"weight": 87.8, "delay": 0.1, "receptor": 0})
-    log("  connections in %.1f s (%d synapses)" % (time.time() - t, n_syn))
+    BUILD["connect_s"] = time.time() - t
+    log("  connections in %.1f s (%d synapses)" % (BUILD["connect_s"], n_syn))
     return neuron, n_syn
 
 
@@ -216,7 +221,8 @@ def build_c3(ngpu, scale=1.0, seed=4321, k=500, w_in=0.25, w_p=0.3,
                  {"weight": w_in, "delay": d, "receptor": 1})
     ngpu.Connect(pg, neuron, {"rule": "one_to_one"},
                  {"weight": w_p, "delay": 0.1, "receptor": 0})
-    log("  connections in %.1f s" % (time.time() - t))
+    BUILD["connect_s"] = time.time() - t
+    log("  connections in %.1f s" % BUILD["connect_s"])
     return neuron, n * k
 
 
@@ -226,7 +232,8 @@ def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps,
     ngpu.ActivateSpikeCount(neuron)
     t = time.time()
     ngpu.Calibrate()
-    log("  calibrate (CSR build + upload) in %.1f s" % (time.time() - t))
+    t_cal = time.time() - t
+    log("  calibrate (CSR build + upload) in %.1f s" % t_cal)
     if warmup > 0:
         ngpu.Simulate(warmup * dt)
     extra = after_warmup() if after_warmup else None
@@ -289,6 +296,11 @@ def measure(ngpu, neuron, n_syn, steps, warmup, timing_steps,
     res["n_neurons"] = neuron.n
     res["n_synapses"] = n_syn
     res["steps"] = steps
+    # network construction: Connect calls (host checks; the rules are
+    # generated on the device at calibration) and calibration
+    c_s = BUILD.pop("connect_s", None)
+    res["connect_s"] = None if c_s is None else round(c_s, 3)
+    res["calibrate_s"] = round(t_cal, 3)
     if extra is not None:
         res["after_warmup"] = extra
     return res
@@ -548,6 +560,7 @@ def main():
                      "frac": round(c1["rk5_kernel"]["tflops"] /
                                    FP32_PEAK_TFLOPS, 4)},
         "roofline": roofline(c1, profiled_traffic()),
+        "build_s": {"connect": c1["connect_s"], "calibrate": c1["calibrate_s"]},
     }
     if not args.no_readme:
         ngpu.ResetKernel()
@@ -574,7 +587,9 @@ def main():
             "reference_hardware": "RTX 2080 Ti (README.md:11)",
             "speedup_vs_reference": round(70.0 / rd["wall_s_per_bio_s"], 2),
             "kernels": rd["kernel_ms"],
-            "roofline": roofline(rd)}
+            "roofline": roofline(rd),
+            "build_s": {"connect": rd["connect_s"],
+                        "calibrate": rd["calibrate_s"]}}
         ngpu.ResetKernel()
     if "c2" in args.workloads.split(","):
         log("building c2 (%.3g x 77169 iaf_psc_exp neurons)" % args.c2_scale)
@@ -593,7 +608,9 @@ def main():
             "syn_events_per_s": round(r2["syn_events_per_s"], 1),
             "rate_hz": round(r2["rate_hz"], 3),
             "kernels": r2["kernel_ms"],
-            "roofline": roofline(r2)}
+            "roofline": roofline(r2),
+            "build_s": {"connect": r2["connect_s"],
+                        "calibrate": r2["calibrate_s"]}}
         ngpu.ResetKernel()
     if "c3" in args.workloads.split(","):
         log("building c3 (%.3g x 2e5 aeif_cond_alpha neurons, STDP)"
@@ -624,7 +641,9 @@ def main():
             "rk5_stages_per_neuron_step": round(
                 r3["rk5_stages_per_neuron_step"], 4),
             "kernels": r3["kernel_ms"],
-            "roofline": roofline(r3)}
+            "roofline": roofline(r3),
+            "build_s": {"connect": r3["connect_s"],
+                        "calibrate": r3["calibrate_s"]}}
         ngpu.ResetKernel()
     if "c4" in args.workloads.split(","):
         from neurongpu_amd import workloads as wl
@@ -636,7 +655,8 @@ def main():
             % (len(inst), sum(x["n"] for x in inst)))
         t = time.time()
         neuron, ranges, n_syn = wl.build_ensemble(ngpu, inst)
-        log("  connections in %.1f s" % (time.time() - t))
+        BUILD["connect_s"] = time.time() - t
+        log("  connections in %.1f s" % BUILD["connect_s"])
         r4 = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
                      args.timing_steps,
                      after_warmup=lambda: autotune(ngpu, steps=40, log=log))
@@ -661,7 +681,9 @@ def main():
             "rk5_stages_per_neuron_step": round(
                 r4["rk5_stages_per_neuron_step"], 4),
             "kernels": r4["kernel_ms"],
-            "roofline": roofline(r4)}
+            "roofline": roofline(r4),
+            "build_s": {"connect": r4["connect_s"],
+                        "calibrate": r4["calibrate_s"]}}
         ngpu.ResetKernel()
     if not args.no_cpu_baseline:
         log("cpu baseline")

@@ -256,6 +256,17 @@ def test_error_port_out_of_range(ngpu):
                      {"weight": 1.0, "delay": 1.0, "receptor": 2})
 
 
+@pytest.mark.parametrize("conn", [
+    {"rule": "fixed_indegree", "indegree": -1},
+    {"rule": "fixed_outdegree", "outdegree": -1},
+    {"rule": "fixed_total_number", "total_num": -1},
+])
+def test_error_negative_connection_count(ngpu, conn):
+    a = ngpu.Create("aeif_cond_beta", 4)
+    with pytest.raises(ValueError):
+        ngpu.Connect(a, a, conn, {"weight": 1.0, "delay": 1.0})
+
+
 def test_error_unknown_model(ngpu):
     with pytest.raises(ValueError):
         ngpu.Create("no_such_model", 3)

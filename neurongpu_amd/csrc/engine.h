@@ -422,9 +422,11 @@ class Engine {
                    long long it_local);
   void EnsureHostCSR();
   void BuildDeviceStore();
-  // canonical rows of one source: delay segment offsets and records
-  void SourceRows(int s, std::vector<uint32_t> &seg,
-                  std::vector<unsigned long long> &rec);
+  // the rows of sources s0 .. s0+n-1 in three copies: base[0..n] (offsets
+  // into rec, relative to base[0]), seg[(D+1)*i ..] and rec
+  void SourceRangeRows(int s0, int n, std::vector<unsigned long long> &base,
+                       std::vector<uint32_t> &seg,
+                       std::vector<unsigned long long> &rec);
   FoldIn MakeFold(long long pois_T, double pois_t);
   SpikeOut MakeSpikeOut(Group &g, long long T, double t);
 };

@@ -141,6 +141,10 @@ struct NeuronGPUStats {
   unsigned long long syn_events;
   unsigned long long n_spikes;
   double step_seconds;   // wall time inside SimulationStep loops
+  // the part of rk5_accepted / rk5_rejected made by the first pass of the
+  // two-pass AdEx update (at most rk5_budget attempts per neuron-step)
+  unsigned long long rk5_pass1_accepted;
+  unsigned long long rk5_pass1_rejected;
 };
 
 class NeuronGPU {

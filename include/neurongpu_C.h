@@ -190,6 +190,11 @@ extern "C" {
    * out[1] RK5 rejected attempts, out[2] synaptic events delivered. */
   int NeuronGPU_GetHotPathStats(unsigned long long *out3);
 
+  /* The first pass's share of those counts (two-pass AdEx update, at most
+   * rk5_budget attempts per neuron-step): out[0] accepted, out[1]
+   * rejected. */
+  int NeuronGPU_GetRk5Pass1Stats(unsigned long long *out2);
+
   /* Per-kernel device time measured with HIP events on the engine's stream
    * (on != 0 enables; enabling or disabling clears the totals).  Kinds:
    * 0 AdEx RK5 update (first pass), 1 iaf_psc_exp update, 2 spike

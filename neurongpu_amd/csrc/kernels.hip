@@ -588,7 +588,7 @@ void aeif_pass1_kernel(AeifArgs A) {
     }
   }
   emit_spikes(A.out, sn, active && finished, ii, node, nsp, (float)nsp);
-  aeif_count<P>(A, n_acc, n_rej);
+  rk5_count(A.cnt1 ? A.cnt1 : A.cnt, n_acc, n_rej);
   if (OV && A.mask_write) {
     const unsigned long long m =
         __ballot(slot != 0xFFFFFFFFu || cslot != 0xFFFFFFFFu);

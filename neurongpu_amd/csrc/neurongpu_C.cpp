@@ -447,6 +447,13 @@ int NeuronGPU_GetHotPathStats(unsigned long long *out3)
       out3[2] = s.syn_events;
     })
 
+int NeuronGPU_GetRk5Pass1Stats(unsigned long long *out2)
+    RET(int, 0, {
+      NeuronGPUStats s = instance->GetStats();
+      out2[0] = s.rk5_pass1_accepted;
+      out2[1] = s.rk5_pass1_rejected;
+    })
+
 }  // extern "C"
 
 extern "C" {

@@ -157,6 +157,7 @@ struct AeifArgs {
   FoldIn fold;
   SpikeOut out;
   unsigned long long *cnt;    // [kCounterSlots][2] accepted, rejected
+  unsigned long long *cnt1;   // the same for the first pass (nullptr: cnt)
   int do_step;                // 0: fold only (end-of-simulation flush)
   // two-pass update: first pass with at most `budget` RK5 attempts per
   // neuron, the rest finished by a second pass over 64 sub-lists of

@@ -185,6 +185,7 @@ _SIG = {
     "RemoteCreate": (c_int, (c_int, c_char_p, c_int, c_int)),
     "ResetKernel": (c_int, ()),
     "GetHotPathStats": (c_int, (ctypes.POINTER(ctypes.c_ulonglong),)),
+    "GetRk5Pass1Stats": (c_int, (ctypes.POINTER(ctypes.c_ulonglong),)),
     "SetKernelTiming": (c_int, (c_int,)),
     "GetKernelTimes": (c_int, (ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_longlong))),
@@ -1406,6 +1407,13 @@ def GetHotPathStats():
     a = (ctypes.c_ulonglong * 3)()
     _call("GetHotPathStats", a)
     return {"rk5_accepted": a[0], "rk5_rejected": a[1], "syn_events": a[2]}
+
+
+def GetRk5Pass1Stats():
+    "The first RK5 pass's share of the accepted/rejected counts"
+    a = (ctypes.c_ulonglong * 2)()
+    _call("GetRk5Pass1Stats", a)
+    return {"rk5_pass1_accepted": a[0], "rk5_pass1_rejected": a[1]}
 
 
 KERNEL_KINDS = ("aeif_rk5", "iaf_psc_exp", "relay", "spike_generator",

@@ -459,6 +459,33 @@ def test_step_overlap_transitions(ngpu):
     assert got == ref
 
 
+def test_overlapped_steps_then_graph_steps(ngpu):
+    """Overlapped binned steps from the stepping API (the last one leaves
+    its second pass running), then captured steps switched on with no host
+    synchronisation in between, then Simulate: the spikes equal those of
+    plain eager steps."""
+    log("case overlap_then_graph_steps")
+    spec = nb.ensemble_spec(_ensemble_instances())
+
+    def run(first, then):
+        groups, _ = nb.build_engine(ngpu, spec)
+        ngpu.ActivateRecSpikeTimes(groups[0], 2000)
+        for k, v in first.items():
+            ngpu.SetKernelStatus(k, v)
+        ngpu.StartSimulation()
+        for _ in range(700):
+            ngpu.SimulationStep()
+        for k, v in then.items():
+            ngpu.SetKernelStatus(k, v)
+        ngpu.Simulate(130.0)
+        return sorted(nb.engine_spikes(ngpu, groups[0]))
+
+    ref = run({"step_overlap": 0}, {})
+    got = run({"delivery_mode": 1}, {"graph_steps": 16})
+    assert len(ref) > 100
+    assert got == ref
+
+
 def _conn_table(ngpu, n):
     ids = ngpu.GetConnections(ngpu.NodeSeq(0, n), ngpu.NodeSeq(0, n))
     out = []

@@ -75,43 +75,12 @@ def hip_sync():
 
 
 def build_c1(ngpu, n=1000000, seed=1234, tol=1e-3):
-    rng = np.random.default_rng(seed)
-    ngpu.ResetKernel()
-    ngpu.SetVerbosityLevel(0)
-    ngpu.SetRandomSeed(seed)
-    n_e = int(round(0.8 * n))
-    neuron = ngpu.Create("aeif_cond_beta", n, 2)
-    ngpu.SetStatus(neuron, {
-        "C_m": 281.0, "g_L": 30.0, "E_L": -70.6, "V_th": -50.4,
-        "Delta_T": 2.0, "a": 4.0, "tau_w": 144.0, "b": 80.5,
-        "V_reset": -60.0, "t_ref": 0.0, "E_rev": [0.0, -85.0],
-        "tau_rise": [0.2, 0.5], "tau_decay": [2.0, 10.0]})
-    ngpu.SetStatus(neuron, {"rk5_tol": tol})
+    from neurongpu_amd import workloads as wl
     t = time.time()
-    v0 = rng.uniform(-70.0, -55.0, n).astype(np.float32)
-    ngpu.SetStatus(neuron, {"V_m": {"array": v0}})
-    log("  V0 set in %.1f s" % (time.time() - t))
-    pg = ngpu.Create("poisson_generator", 1)
-    ngpu.SetStatus(pg, {"rate": 8000.0})
-    exc = neuron[0:n_e]
-    inh = neuron[n_e:n]
-    k_e, k_i = 800, 200
-    t = time.time()
-    d_e = (rng.integers(1, 21, n * k_e, dtype=np.int8).astype(np.float32)
-           * np.float32(0.1))
-    ngpu.Connect(exc, neuron, {"rule": "fixed_indegree", "indegree": k_e},
-                 {"weight": 0.05, "delay": {"array": d_e}, "receptor": 0})
-    del d_e
-    d_i = (rng.integers(1, 21, n * k_i, dtype=np.int8).astype(np.float32)
-           * np.float32(0.1))
-    ngpu.Connect(inh, neuron, {"rule": "fixed_indegree", "indegree": k_i},
-                 {"weight": 0.25, "delay": {"array": d_i}, "receptor": 1})
-    del d_i
-    ngpu.Connect(pg, neuron, {"rule": "all_to_all"},
-                 {"weight": 0.1, "delay": 0.1, "receptor": 0})
+    neuron, n_syn, _ = wl.build_c1(ngpu, n=n, seed=seed, tol=tol)
     BUILD["connect_s"] = time.time() - t
-    log("  connections in %.1f s" % BUILD["connect_s"])
-    return neuron, n * (k_e + k_i)
+    log("  network built in %.1f s" % BUILD["connect_s"])
+    return neuron, n_syn
 
 
 def build_readme(ngpu, n=1000000, seed=1234, tol=None):
@@ -133,16 +102,15 @@ def build_readme(ngpu, n=1000000, seed=1234, tol=None):
     if tol is not None:
         ngpu.SetStatus(neuron, {"rk5_tol": tol})
     t = time.time()
-    d_e = ngpu.RandomNormalClipped(ce * (n_e + n_i), 0.5, 0.25, 0.1,
-                                   0.5 + 3 * 0.25)
+    # delays N(0.5, 0.25) ms clipped to [0.1, 0.5 + 3 * 0.25], drawn per
+    # synapse by the library (brunel_net.cpp draws them with
+    # RandomNormalClipped into host arrays)
+    delay = {"distribution": "normal_clipped", "mu": 0.5, "sigma": 0.25,
+             "low": 0.1, "high": 0.5 + 3 * 0.25}
     ngpu.Connect(exc, neuron, {"rule": "fixed_indegree", "indegree": ce},
-                 {"weight": 0.05, "delay": {"array": d_e}, "receptor": 0})
-    del d_e
-    d_i = ngpu.RandomNormalClipped(ci * (n_e + n_i), 0.5, 0.25, 0.1,
-                                   0.5 + 3 * 0.25)
+                 {"weight": 0.05, "delay": delay, "receptor": 0})
     ngpu.Connect(inh, neuron, {"rule": "fixed_indegree", "indegree": ci},
-                 {"weight": 0.35, "delay": {"array": d_i}, "receptor": 1})
-    del d_i
+                 {"weight": 0.35, "delay": delay, "receptor": 1})
     ngpu.Connect(pg, neuron, {"rule": "all_to_all"},
                  {"weight": 0.37, "delay": 0.2, "receptor": 0})
     BUILD["connect_s"] = time.time() - t

@@ -105,6 +105,11 @@ class ConnSpec {
 class SynSpec {
   unsigned char syn_group_;
   unsigned char port_;
+  // distribution parameters of weight / delay (no reference counterpart):
+  // mu, sigma, low, high, step; weight_distr_ / delay_distr_ select the
+  // distribution (0 none, 1 uniform, 2 normal, 3 normal_clipped)
+  float weight_dp_[5];
+  float delay_dp_[5];
 
  public:
   int weight_distr_;
@@ -145,6 +150,9 @@ struct NeuronGPUStats {
   // two-pass AdEx update (at most rk5_budget attempts per neuron-step)
   unsigned long long rk5_pass1_accepted;
   unsigned long long rk5_pass1_rejected;
+  // accepted attempts whose scaled error exceeded 1 (the acceptance test
+  // lets them pass only at the minimum step size)
+  unsigned long long rk5_forced;
 };
 
 class NeuronGPU {
@@ -480,6 +488,17 @@ class NeuronGPU {
   // generator, 4 delivery, 5 post-synaptic STDP, 6 record)
   int SetKernelTiming(int on);
   int GetKernelTimes(double *ms8, long long *cnt8);
+  // no reference counterpart: bulk connection queries (canonical order of
+  // GetConnections; n_source / n_target <= 0 mean every node) and synapse
+  // counts per node and delay slot
+  long long GetConnectionArrays(int i_source, int n_source, int i_target,
+                                int n_target, long long n_max, int *source,
+                                int *target, unsigned char *port,
+                                unsigned char *syn_group, float *delay,
+                                float *weight);
+  int GetConnectionCounts(int i_node, int n_node, int port, int *in_degree,
+                          int *out_degree, long long *delay_count,
+                          int n_delay);
   // no reference counterpart: learned RK5 step-size proposer
   int ProposerLog(int on, long long max_samples);
   long long ProposerNSamples();

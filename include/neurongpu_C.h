@@ -195,6 +195,12 @@ extern "C" {
    * rejected. */
   int NeuronGPU_GetRk5Pass1Stats(unsigned long long *out2);
 
+  /* RK5 attempts accepted with a scaled error above 1 since calibration:
+   * the acceptance test lets them pass only at the minimum step size
+   * (src/rk5.h:125), whatever step size the controller or the learned
+   * proposer chose. */
+  int NeuronGPU_GetRk5ForcedAccepts(unsigned long long *out1);
+
   /* Per-kernel device time measured with HIP events on the engine's stream
    * (on != 0 enables; enabling or disabling clears the totals).  Kinds:
    * 0 AdEx RK5 update (first pass), 1 iaf_psc_exp update, 2 spike
@@ -203,6 +209,29 @@ extern "C" {
    * launch). */
   int NeuronGPU_SetKernelTiming(int on);
   int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8);
+
+  /* Bulk form of GetConnections + GetConnectionStatus: the connections
+   * with source in [i_source, i_source + n_source) and target in
+   * [i_target, i_target + n_target) (n <= 0: every node), in the canonical
+   * order of GetConnections.  Returns their number; the first n_max are
+   * written into the caller's arrays (any may be NULL). */
+  long long NeuronGPU_GetConnectionArrays(int i_source, int n_source,
+                                          int i_target, int n_target,
+                                          long long n_max, int *source,
+                                          int *target, unsigned char *port,
+                                          unsigned char *syn_group,
+                                          float *delay, float *weight);
+
+  /* Synapse counts for nodes i_node .. i_node + n_node - 1, counted on the
+   * device after calibration: in_degree[i] / out_degree[i] = synapses with
+   * node i_node + i as target / source, delay_count[d] = synapses with a
+   * source in the range and a delay of d + 1 time steps (d < n_delay).
+   * Only synapses on receptor `port` count (every port if port < 0).
+   * Poisson generators' direct connections are not synapses of the store
+   * and are not counted.  Any output may be NULL. */
+  int NeuronGPU_GetConnectionCounts(int i_node, int n_node, int port,
+                                    int *in_degree, int *out_degree,
+                                    long long *delay_count, int n_delay);
 
   /* Learned RK5 step-size proposer (6-32-32-1 MLP, BF16 inference, FP32
    * training), used for the RK5 steps of the second pass of the AdEx update

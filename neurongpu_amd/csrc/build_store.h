@@ -25,4 +25,16 @@ const char *BuildReverseIndex(const unsigned long long *rec,
                               uint32_t **rev_idx_out,
                               unsigned long long *n_rev_out, void *stream);
 
+// Synapse counts of the canonical store (device pointers): for nodes lo ..
+// hi-1, in_deg[t - lo] += synapses with target t, out_deg[s - lo] +=
+// synapses with source s, delay_cnt[d] += synapses with source in the range
+// and delay index d < n_delay; only synapses on receptor `port` (all ports
+// if port < 0).  Outputs may be null and are added to (the caller zeroes
+// them).  Enqueued on `stream`.
+void CountSynapses(const unsigned long long *rec,
+                   const unsigned long long *base, const uint32_t *seg,
+                   uint32_t N, uint32_t D, int lo, int hi, int port,
+                   int *in_deg, int *out_deg, unsigned long long *delay_cnt,
+                   int n_delay, void *stream);
+
 }  // namespace ngpu

@@ -67,6 +67,45 @@ __global__ void rev_keys(const unsigned long long *rec, const uint8_t *sg,
   atomicAdd(count + t, 1u);
 }
 
+// Connection statistics: one wavefront per source walks its records in
+// delay order.  Per-source and per-delay counts are wave sums (a source is
+// owned by one wave); in-degrees are atomics on the target's counter.
+__global__ __launch_bounds__(256) void count_synapses(
+    const unsigned long long *rec, const unsigned long long *base,
+    const uint32_t *seg, uint32_t s_begin, uint32_t s_end, uint32_t D,
+    int lo, int hi, int port, int *in_deg, int *out_deg,
+    unsigned long long *delay_cnt, int n_delay) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n_wave = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t s = s_begin + ((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+       s < s_end; s += n_wave) {
+    const bool own = (int)s >= lo && (int)s < hi;  // wave-uniform
+    const unsigned long long *r = rec + base[s];
+    const uint32_t *sg = seg + (unsigned long long)s * (D + 1);
+    int n_own = 0;
+    for (uint32_t d = 0; d < D; d++) {
+      const uint32_t j1 = sg[d + 1];
+      int n_d = 0;
+      for (uint32_t j = sg[d] + lane; j < j1; j += 64u) {
+        const uint32_t tp = (uint32_t)r[j];
+        if (port >= 0 && (int)(tp >> kPortShift) != port) continue;
+        n_d++;
+        const uint32_t t = tp & kTargetMask;
+        if (in_deg && (int)t >= lo && (int)t < hi)
+          atomicAdd(in_deg + (t - (uint32_t)lo), 1);
+      }
+      if (own) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) n_d += __shfl_xor(n_d, off);
+        n_own += n_d;
+        if (delay_cnt && (int)d < n_delay && lane == 0 && n_d > 0)
+          atomicAdd(delay_cnt + d, (unsigned long long)n_d);
+      }
+    }
+    if (own && out_deg && lane == 0) out_deg[s - (uint32_t)lo] += n_own;
+  }
+}
+
 template <class T>
 struct DTmp {
   T *p = nullptr;
@@ -186,6 +225,23 @@ const char *BuildReverseIndex(const unsigned long long *rec,
   *rev_idx_out = idx;
   *n_rev_out = n_rev;
   return nullptr;
+}
+
+void CountSynapses(const unsigned long long *rec,
+                   const unsigned long long *base, const uint32_t *seg,
+                   uint32_t N, uint32_t D, int lo, int hi, int port,
+                   int *in_deg, int *out_deg, unsigned long long *delay_cnt,
+                   int n_delay, void *stream) {
+  // in-degrees need every source; the other counts only the range's own
+  const uint32_t s_begin = in_deg ? 0u : (uint32_t)lo;
+  const uint32_t s_end = in_deg ? N : (uint32_t)hi;
+  if (s_end <= s_begin || D == 0) return;
+  unsigned long long waves = s_end - s_begin;
+  unsigned int grid = (unsigned int)((waves + 3) / 4);
+  if (grid > 65536u) grid = 65536u;
+  hipLaunchKernelGGL(count_synapses, dim3(grid), dim3(256), 0,
+                     (hipStream_t)stream, rec, base, seg, s_begin, s_end, D,
+                     lo, hi, port, in_deg, out_deg, delay_cnt, n_delay);
 }
 
 }  // namespace ngpu

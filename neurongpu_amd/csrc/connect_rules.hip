@@ -61,8 +61,12 @@ __device__ __forceinline__ int node_t(const ConnGenArgs &a, uint32_t k) {
 // delay of insertion index k)
 __device__ __forceinline__ void put(const ConnGenArgs &a, unsigned long long k,
                                     int s, int t) {
-  const float wv = a.w_arr ? a.w_arr[k] : a.w0;
-  const float dv = a.d_arr ? a.d_arr[k] : a.d0;
+  const float wv = a.w_arr ? a.w_arr[k]
+                           : (a.wd.kind != kDistNone ? dist_sample(a.wd, k)
+                                                     : a.w0);
+  const float dv = a.d_arr ? a.d_arr[k]
+                           : (a.dd.kind != kDistNone ? dist_sample(a.dd, k)
+                                                     : a.d0);
   a.src[k] = (uint32_t)s;
   a.tp[k] = (uint32_t)t | a.port_word;
   a.w[k] = wv;

@@ -222,6 +222,7 @@ class Engine {
     int port = 0, sg = 0;
     float w0 = 0.0f, d0 = 0.0f;
     std::vector<float> w_arr, d_arr;
+    DistSpec wd{}, dd{};          // distributions (kind kDistNone: none)
     unsigned long long seed = 0, call = 0;
   };
   // connections in coordinate form and insertion order, one chunk per
@@ -367,6 +368,12 @@ class Engine {
   DBuf<uint32_t> d_dir_ptr, d_dir_canon, d_dir_gen, d_gen_table, d_tables;
   DBuf<float> d_dir_w, d_dir_delay, d_gen_origin, d_gen_start, d_gen_stop;
   DBuf<uint8_t> d_dir_port;
+  // compact form of the direct connections (DirectIn::compact)
+  bool dir_compact = false;
+  uint32_t dir_lo = 0, dir_hi = 0, dir_gen0 = 0;
+  int dir_gen_mode = 0, dir_port = 0;
+  float dir_w = 0.0f, dir_delay = 0.0f;
+  DBuf<uint32_t> d_dir_gen_of;
   bool pois_dirty = true;
   // the group of CreatePoissonGenerator (src/poisson.cu:98-147): mean count
   // per step, its CDF table, and the count above which a draw is dropped

@@ -108,9 +108,10 @@ __device__ __forceinline__ bool aeif_rk5_step(const AeifLocal<P> &p, float refr,
                                               float &h, float h_min,
                                               double h_min_acc, float h_max,
                                               float eps, int &n_att,
-                                              int &n_acc, int &n_rej) {
+                                              int &n_acc, int &n_rej,
+                                              unsigned long long *forced) {
   return rk5_step(AeifModel<P>{p}, refr, x, y, h, h_min, h_min_acc, h_max,
-                  eps, n_att, n_acc, n_rej);
+                  eps, n_att, n_acc, n_rej, forced);
 }
 
 template <int P>
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void aeif_kernel(AeifArgs A) {
       while (!end) {
         float hmax = (float)(x1 - x);
         if (!aeif_rk5_step<P>(p, refr, x, y, h, A.h_min, A.h_min_acc, hmax,
-                              A.eps, n_att, n_acc, n_rej))
+                              A.eps, n_att, n_acc, n_rej, A.forced))
           x = x1;
         end = (x >= x1 - (double)A.h_min);
         // ExternalUpdate
@@ -419,6 +420,7 @@ __device__ __forceinline__ bool aeif_attempt_and_update(
   n_att++;
   if (err <= 1.0f || (double)hh <= A.h_min_acc) {
     n_acc++;
+    if (err > 1.0f) count_forced(A.forced);
     x += (double)h;
     if (err > CK::err_min) {
       h = (h * CK::coeff) * dev_pow_inc(err);

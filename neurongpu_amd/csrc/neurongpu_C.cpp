@@ -447,6 +447,9 @@ int NeuronGPU_GetHotPathStats(unsigned long long *out3)
       out3[2] = s.syn_events;
     })
 
+int NeuronGPU_GetRk5ForcedAccepts(unsigned long long *out1)
+    RET(int, 0, out1[0] = instance->GetStats().rk5_forced;)
+
 int NeuronGPU_GetRk5Pass1Stats(unsigned long long *out2)
     RET(int, 0, {
       NeuronGPUStats s = instance->GetStats();
@@ -461,6 +464,23 @@ int NeuronGPU_SetKernelTiming(int on)
     RET(int, 0, ret = instance->SetKernelTiming(on))
 int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8)
     RET(int, 0, ret = instance->GetKernelTimes(ms8, cnt8))
+long long NeuronGPU_GetConnectionArrays(int i_source, int n_source,
+                                        int i_target, int n_target,
+                                        long long n_max, int *source,
+                                        int *target, unsigned char *port,
+                                        unsigned char *syn_group,
+                                        float *delay, float *weight)
+    RET(long long, 0,
+        ret = instance->GetConnectionArrays(i_source, n_source, i_target,
+                                            n_target, n_max, source, target,
+                                            port, syn_group, delay, weight))
+int NeuronGPU_GetConnectionCounts(int i_node, int n_node, int port,
+                                  int *in_degree, int *out_degree,
+                                  long long *delay_count, int n_delay)
+    RET(int, 0,
+        ret = instance->GetConnectionCounts(i_node, n_node, port, in_degree,
+                                            out_degree, delay_count,
+                                            n_delay))
 int NeuronGPU_ProposerLog(int on, long long max_samples)
     RET(int, 0, ret = instance->ProposerLog(on, max_samples))
 long long NeuronGPU_ProposerNSamples()

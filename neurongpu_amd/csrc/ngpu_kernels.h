@@ -14,6 +14,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "conn_dist.h"
+
 namespace ngpu {
 
 constexpr int kMaxPort = 20;          // src/aeif_cond_beta.h:26
@@ -50,6 +52,7 @@ struct ConnGenArgs {
   uint8_t sg_val;
   float w0, d0, dt;
   const float *w_arr, *d_arr;   // per connection, or nullptr
+  DistSpec wd, dd;              // else drawn per connection (kind != 0)
   uint32_t k0, k1, call;        // stream key and call index
   unsigned long long m;         // connections
   uint32_t *src, *tp;           // outputs at the chunk's offset
@@ -106,6 +109,18 @@ struct DirectIn {
   const uint32_t *tables;     // [n_table][kPoissonTableMax]
   uint32_t key0, key1;
   int active;                 // any direct connections at all
+  // Compact form (compact != 0), used when nodes lo .. hi-1 each have
+  // exactly one direct connection, in canonical order (canonical index =
+  // node - lo), all with one weight, delay and port: the per-connection
+  // arrays above are not read.  The generator of node lo + j is gen0
+  // (gen_mode 0), gen0 + j (1: one generator per node) or gen_of[j] (2).
+  int compact;
+  uint32_t lo, hi;
+  int gen_mode;
+  uint32_t gen0;
+  const uint32_t *gen_of;
+  float c_weight, c_delay;
+  int c_port;
 };
 
 // Inputs to fold at the start of a node update (end-of-step port input of
@@ -158,6 +173,8 @@ struct AeifArgs {
   SpikeOut out;
   unsigned long long *cnt;    // [kCounterSlots][2] accepted, rejected
   unsigned long long *cnt1;   // the same for the first pass (nullptr: cnt)
+  unsigned long long *forced; // [kCounterSlots] attempts accepted with a
+                              // scaled error above 1 (at the minimum step)
   int do_step;                // 0: fold only (end-of-simulation flush)
   // two-pass update: first pass with at most `budget` RK5 attempts per
   // neuron, the rest finished by a second pass over 64 sub-lists of
@@ -221,6 +238,7 @@ struct Rk5ModelArgs {
   FoldIn fold;
   SpikeOut out;
   unsigned long long *cnt;    // [kCounterSlots][2] accepted, rejected
+  unsigned long long *forced; // [kCounterSlots] accepted above tolerance
   int do_step;                // 0: fold only (end-of-simulation flush)
   const StepClock *clock;
 };

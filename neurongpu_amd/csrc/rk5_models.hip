@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void rk5_model_kernel(Rk5ModelArgs A) {
       while (!end) {
         float hmax = (float)(x1 - x);
         if (!rk5_step(m, refr, x, y, h, A.h_min, A.h_min_acc, hmax, A.eps,
-                      n_att, n_acc, n_rej))
+                      n_att, n_acc, n_rej, A.forced))
           x = x1;
         end = (x >= x1 - (double)A.h_min);
         // ExternalUpdate

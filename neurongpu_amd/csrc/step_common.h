@@ -92,8 +92,36 @@ __device__ __forceinline__ void emit_spikes(const SpikeOut &o,
   }
 }
 
+// Poisson count of one direct connection (canonical index j, generator g)
+// for the pending step: 0 outside the generator's window
+// (src/poiss_gen.cu:43-74), else one Philox draw inverted over the
+// generator's CDF table.
+__device__ __forceinline__ int pois_count(const DirectIn &d, uint32_t j,
+                                          uint32_t g, float delay,
+                                          uint32_t Tlo, uint32_t Thi,
+                                          double pois_t) {
+  double t_rel = pois_t - (double)d.gen_origin[g] - (double)delay;
+  if (!(t_rel >= (double)d.gen_start[g] && t_rel <= (double)d.gen_stop[g]))
+    return 0;
+  uint32_t ti = d.gen_table[g];
+  if (ti == 0xFFFFFFFFu) return 0;
+  const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
+  uint32_t u = dev_philox_w0(j, Tlo, Thi, kPoissonTag, d.key0, d.key1);
+  int k = 0;
+  while (u > tab[k]) k++;
+  return k;
+}
+
+// Generator of the compact form's connection j.
+__device__ __forceinline__ uint32_t compact_gen(const DirectIn &d,
+                                                uint32_t j) {
+  return d.gen_mode == 0 ? d.gen0
+                         : (d.gen_mode == 1 ? d.gen0 + j : d.gen_of[j]);
+}
+
 // Port input of the previous step: delivery accumulator of every port plus
 // the Poisson draws of the node's direct connections (canonical order).
+// An accumulator entry that is already 0 is not written back.
 template <int P>
 __device__ __forceinline__ void gather_input(const FoldIn &f,
                                              const StepNow &sn, int node,
@@ -102,24 +130,28 @@ __device__ __forceinline__ void gather_input(const FoldIn &f,
   for (int p = 0; p < P; p++) {
     float *ap = f.acc + (long long)p * f.n_total + node;
     a[p] = *ap;
-    *ap = 0.0f;
+    if (a[p] != 0.0f) *ap = 0.0f;
   }
   if (sn.pois_T < 0 || !f.dir.active) return;
   const DirectIn &d = f.dir;
-  uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
   uint32_t Tlo = (uint32_t)sn.pois_T, Thi = (uint32_t)((uint64_t)sn.pois_T >> 32);
+  if (d.compact) {
+    const uint32_t j = (uint32_t)node - d.lo;
+    if (j >= d.hi - d.lo) return;
+    const int k = pois_count(d, j, compact_gen(d, j), d.c_delay, Tlo, Thi,
+                             sn.pois_t);
+    if (k > 0) {
+      float add = d.c_weight * (float)k;
+#pragma unroll
+      for (int p = 0; p < P; p++)
+        if (d.c_port == p) a[p] = a[p] + add;
+    }
+    return;
+  }
+  uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
   for (uint32_t j = j0; j < j1; j++) {
-    uint32_t g = d.gen[j];
-    double t_rel = sn.pois_t - (double)d.gen_origin[g] - (double)d.delay[j];
-    if (!(t_rel >= (double)d.gen_start[g] && t_rel <= (double)d.gen_stop[g]))
-      continue;
-    uint32_t ti = d.gen_table[g];
-    if (ti == 0xFFFFFFFFu) continue;
-    const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
-    uint32_t u = dev_philox_w0(d.canon[j], Tlo, Thi, kPoissonTag, d.key0,
-                               d.key1);
-    int k = 0;
-    while (u > tab[k]) k++;
+    const int k = pois_count(d, d.canon[j], d.gen[j], d.delay[j], Tlo, Thi,
+                             sn.pois_t);
     if (k > 0) {
       float add = d.weight[j] * (float)k;
       int port = d.port[j];
@@ -137,24 +169,23 @@ __device__ __forceinline__ float gather_port(const FoldIn &f, long long pois_T,
                                              double pois_t, int node, int p) {
   float *ap = f.acc + (long long)p * f.n_total + node;
   float a = *ap;
-  *ap = 0.0f;
+  if (a != 0.0f) *ap = 0.0f;
   if (pois_T < 0 || !f.dir.active) return a;
   const DirectIn &d = f.dir;
-  uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
   uint32_t Tlo = (uint32_t)pois_T, Thi = (uint32_t)((uint64_t)pois_T >> 32);
+  if (d.compact) {
+    const uint32_t j = (uint32_t)node - d.lo;
+    if (j >= d.hi - d.lo || d.c_port != p) return a;
+    const int k =
+        pois_count(d, j, compact_gen(d, j), d.c_delay, Tlo, Thi, pois_t);
+    if (k > 0) a = a + d.c_weight * (float)k;
+    return a;
+  }
+  uint32_t j0 = d.ptr[node], j1 = d.ptr[node + 1];
   for (uint32_t j = j0; j < j1; j++) {
     if ((int)d.port[j] != p) continue;
-    uint32_t g = d.gen[j];
-    double t_rel = pois_t - (double)d.gen_origin[g] - (double)d.delay[j];
-    if (!(t_rel >= (double)d.gen_start[g] && t_rel <= (double)d.gen_stop[g]))
-      continue;
-    uint32_t ti = d.gen_table[g];
-    if (ti == 0xFFFFFFFFu) continue;
-    const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
-    uint32_t u = dev_philox_w0(d.canon[j], Tlo, Thi, kPoissonTag, d.key0,
-                               d.key1);
-    int k = 0;
-    while (u > tab[k]) k++;
+    const int k = pois_count(d, d.canon[j], d.gen[j], d.delay[j], Tlo, Thi,
+                             pois_t);
     if (k > 0) a = a + d.weight[j] * (float)k;
   }
   return a;
@@ -249,6 +280,12 @@ __device__ __forceinline__ float rk5_attempt(
   return err;
 }
 
+// An attempt accepted with a scaled error above 1, which the acceptance
+// test allows only at the minimum step size (rare: one atomic each).
+__device__ __forceinline__ void count_forced(unsigned long long *forced) {
+  if (forced) atomicAdd(forced + (blockIdx.x & (kCounterSlots - 1)), 1ull);
+}
+
 // One adaptive Cash-Karp step (src/rk5.h:57-147).  Returns false when the
 // attempt cap is reached.
 template <class M>
@@ -256,7 +293,8 @@ __device__ __forceinline__ bool rk5_step(const M &m, float refr, double &x,
                                          float (&y)[M::N], float &h,
                                          float h_min, double h_min_acc,
                                          float h_max, float eps, int &n_att,
-                                         int &n_acc, int &n_rej) {
+                                         int &n_acc, int &n_rej,
+                                         unsigned long long *forced) {
   constexpr int N = M::N;
   float k1[N], yn[N], inv_scal[N];
   float err = 0.0f;
@@ -277,6 +315,7 @@ __device__ __forceinline__ bool rk5_step(const M &m, float refr, double &x,
     h = (h_new > h_lo) ? h_new : h_lo;
   }
   n_acc++;
+  if (err > 1.0f) count_forced(forced);
   x += (double)h;
   if (err > CK::err_min) {
     h = (h * CK::coeff) * dev_pow_inc(err);

@@ -186,6 +186,7 @@ _SIG = {
     "ResetKernel": (c_int, ()),
     "GetHotPathStats": (c_int, (ctypes.POINTER(ctypes.c_ulonglong),)),
     "GetRk5Pass1Stats": (c_int, (ctypes.POINTER(ctypes.c_ulonglong),)),
+    "GetRk5ForcedAccepts": (c_int, (ctypes.POINTER(ctypes.c_ulonglong),)),
     "SetKernelTiming": (c_int, (c_int,)),
     "GetKernelTimes": (c_int, (ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_longlong))),
@@ -199,6 +200,12 @@ _SIG = {
     "ProposerSetWeights": (c_int, (c_void_p,)),
     "ProposerUse": (c_int, (c_int,)),
     "ProposerEval": (c_int, (c_void_p, c_int, c_void_p)),
+    "GetConnectionArrays": (ctypes.c_longlong,
+                            (c_int, c_int, c_int, c_int, ctypes.c_longlong,
+                             c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p)),
+    "GetConnectionCounts": (c_int, (c_int, c_int, c_int, c_void_p, c_void_p,
+                                    c_void_p, c_int)),
 }
 
 _C = {}
@@ -769,6 +776,14 @@ def DictToArray(param_dict, array_size):
         return RandomNormal(array_size, mu, sigma)
     if dist_name == "normal_clipped":
         return RandomNormalClipped(array_size, mu, sigma, low, high, vstep)
+    if dist_name == "uniform":
+        u = RandomUniform(array_size)
+        a = (np.float32(low) + np.float32(high - low) * u).astype(np.float32)
+        if vstep > 0:
+            n = int(np.floor((high - low) / vstep + 0.5)) + 1
+            j = np.minimum(np.floor(u * n), n - 1)
+            a = (np.float32(low) + np.float32(vstep) * j).astype(np.float32)
+        return a
     raise ValueError("Unknown distribution")
 
 
@@ -1025,6 +1040,30 @@ def RuleArraySize(conn_dict, source, target):
     raise ValueError("Unknown number of connections for this rule")
 
 
+_DIST_CODE = {"uniform": 1, "normal": 2, "normal_clipped": 3}
+
+
+def _set_syn_distribution(name, d):
+    """A weight or delay distribution of a synapse dictionary ({"distribution":
+    "uniform" | "normal" | "normal_clipped", "mu", "sigma", "low", "high",
+    "step"}) handed to the library, which draws one value per connection
+    (on the device for rules generated there) instead of an array drawn
+    here.  Returns False for an "array" dictionary."""
+    dist = d.get("distribution")
+    if "array" in d or dist is None:
+        return False
+    if dist not in _DIST_CODE:
+        raise ValueError("Unknown distribution")
+    SetSynSpecIntParam(name + "_distribution", _DIST_CODE[dist])
+    for k, v in d.items():
+        if k == "distribution":
+            continue
+        if k not in ("mu", "sigma", "low", "high", "step"):
+            raise ValueError("Unknown parameter name in dictionary")
+        SetSynSpecFloatParam(name + "_" + k, v)
+    return True
+
+
 def _connect_specs(source, target, conn_dict, syn_dict, keep):
     ConnSpecInit()
     SynSpecInit()
@@ -1044,7 +1083,9 @@ def _connect_specs(source, target, conn_dict, syn_dict, keep):
                 v = v.i_syn_group
             SetSynSpecIntParam(k, v)
         elif SynSpecIsFloatParam(k):
-            if isinstance(v, dict):
+            if isinstance(v, dict) and _set_syn_distribution(k, v):
+                pass
+            elif isinstance(v, dict):
                 if not SynSpecIsFloatPtParam(k + "_array"):
                     raise ValueError("Synapse parameter cannot be set by"
                                      " arrays or distributions")
@@ -1406,7 +1447,10 @@ def GetHotPathStats():
     "RK5 accepted/rejected step counts and delivered synaptic events"
     a = (ctypes.c_ulonglong * 3)()
     _call("GetHotPathStats", a)
-    return {"rk5_accepted": a[0], "rk5_rejected": a[1], "syn_events": a[2]}
+    f = (ctypes.c_ulonglong * 1)()
+    _call("GetRk5ForcedAccepts", f)
+    return {"rk5_accepted": a[0], "rk5_rejected": a[1], "syn_events": a[2],
+            "rk5_forced": f[0]}
 
 
 def GetRk5Pass1Stats():
@@ -1414,6 +1458,55 @@ def GetRk5Pass1Stats():
     a = (ctypes.c_ulonglong * 2)()
     _call("GetRk5Pass1Stats", a)
     return {"rk5_pass1_accepted": a[0], "rk5_pass1_rejected": a[1]}
+
+
+def _node_range(nodes):
+    if nodes is None:
+        return 0, 0
+    if isinstance(nodes, NodeSeq):
+        return nodes.i0, nodes.n
+    if isinstance(nodes, int):
+        return nodes, 1
+    raise ValueError("A node range must be a NodeSeq or an int")
+
+
+def GetConnectionArrays(source=None, target=None):
+    """The connections from the node range `source` to the node range
+    `target` (None: every node), in the canonical order of GetConnections,
+    as numpy arrays: {"source", "target", "port", "syn", "delay",
+    "weight"}.  One bulk call instead of one GetConnectionStatus per
+    connection."""
+    s0, ns = _node_range(source)
+    t0, nt = _node_range(target)
+    n = _call("GetConnectionArrays", s0, ns, t0, nt, 0, None, None, None,
+              None, None, None)
+    out = {"source": np.zeros(n, np.int32), "target": np.zeros(n, np.int32),
+           "port": np.zeros(n, np.uint8), "syn": np.zeros(n, np.uint8),
+           "delay": np.zeros(n, np.float32),
+           "weight": np.zeros(n, np.float32)}
+    m = _call("GetConnectionArrays", s0, ns, t0, nt, n,
+              *[out[k].ctypes.data_as(c_void_p) for k in
+                ("source", "target", "port", "syn", "delay", "weight")])
+    if m != n:
+        raise ValueError("The connections changed during the query")
+    return out
+
+
+def GetConnectionCounts(nodes, port=-1, n_delay=0):
+    """Synapse counts of the node range `nodes` (a NodeSeq), counted on the
+    device once the network is calibrated: {"in_degree", "out_degree"}
+    per node and "delay_count"[d] = synapses from the range with a delay of
+    d + 1 time steps (d < n_delay); only synapses on receptor `port`
+    (every port if port < 0)."""
+    i0, n = _node_range(nodes)
+    ind = np.zeros(n, np.int32)
+    outd = np.zeros(n, np.int32)
+    dc = np.zeros(max(n_delay, 1), np.int64)
+    _call("GetConnectionCounts", i0, n, port, ind.ctypes.data_as(c_void_p),
+          outd.ctypes.data_as(c_void_p), dc.ctypes.data_as(c_void_p),
+          n_delay)
+    return {"in_degree": ind, "out_degree": outd,
+            "delay_count": dc[:n_delay]}
 
 
 KERNEL_KINDS = ("aeif_rk5", "iaf_psc_exp", "relay", "spike_generator",

@@ -94,6 +94,55 @@ def _normal_clipped(rng, mu, sigma, low, n, high=np.inf):
 
 
 # ---------------------------------------------------------------------
+# c1: balanced AdEx network at README scale (the headline workload)
+# ---------------------------------------------------------------------
+C1_NEURON = {"C_m": 281.0, "g_L": 30.0, "E_L": -70.6, "V_th": -50.4,
+             "Delta_T": 2.0, "a": 4.0, "tau_w": 144.0, "b": 80.5,
+             "V_reset": -60.0, "t_ref": 0.0, "E_rev": [0.0, -85.0],
+             "tau_rise": [0.2, 0.5], "tau_decay": [2.0, 10.0]}
+
+
+def build_c1(ngpu, n=1000000, seed=1234, tol=1e-3, k_e=800, k_i=200,
+             w_e=0.05, w_i=0.25, w_p=0.1, rate=8000.0, log=None):
+    """n aeif_cond_beta neurons (80 % E), 2 ports (E_rev 0 / -85 mV,
+    tau_rise 0.2 / 0.5 ms, tau_decay 2 / 10 ms), V0 ~ U(-70, -55) mV;
+    fixed in-degree k_e from the E neurons (w_e nS, port 0) plus k_i from
+    the I neurons (w_i nS, port 1), drawn by the library's fixed_indegree
+    rule (on the device at calibration); delays uniform over 1-20 time
+    steps, drawn per synapse by the library (on the device with the rule);
+    one Poisson generator at `rate` Hz connected to every neuron (w_p nS,
+    port 0).  At n = 1e6 this is the 1e9-synapse configuration.  Returns
+    the neurons' node sequence, the synapse count and V0."""
+    import time
+    rng = np.random.default_rng(seed)
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetRandomSeed(seed)
+    n_e = int(round(0.8 * n))
+    neuron = ngpu.Create("aeif_cond_beta", n, 2)
+    ngpu.SetStatus(neuron, dict(C1_NEURON))
+    ngpu.SetStatus(neuron, {"rk5_tol": tol})
+    v0 = rng.uniform(-70.0, -55.0, n).astype(np.float32)
+    ngpu.SetStatus(neuron, {"V_m": {"array": v0}})
+    pg = ngpu.Create("poisson_generator", 1)
+    ngpu.SetStatus(pg, {"rate": rate})
+    t = time.time()
+    delay = {"distribution": "uniform", "low": 0.1, "high": 2.0,
+             "step": 0.1}
+    ngpu.Connect(neuron[0:n_e], neuron,
+                 {"rule": "fixed_indegree", "indegree": k_e},
+                 {"weight": w_e, "delay": delay, "receptor": 0})
+    ngpu.Connect(neuron[n_e:n], neuron,
+                 {"rule": "fixed_indegree", "indegree": k_i},
+                 {"weight": w_i, "delay": delay, "receptor": 1})
+    ngpu.Connect(pg, neuron, {"rule": "all_to_all"},
+                 {"weight": w_p, "delay": 0.1, "receptor": 0})
+    if log:
+        log("  connections in %.1f s" % (time.time() - t))
+    return neuron, n * (k_e + k_i), v0
+
+
+# ---------------------------------------------------------------------
 # c2: layered cortical-column-shaped network of iaf_psc_exp neurons
 # ---------------------------------------------------------------------
 C2_SIZES = [20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948]
@@ -264,8 +313,9 @@ def build_ensemble(ngpu, instances, seed=6464, tol=1e-3, w_p=0.1):
     kernel per step for the whole ensemble): instance k occupies a
     contiguous node range, draws 80 % of its in-degree from its own E
     neurons (port 0) and 20 % from its own I neurons (port 1), delays
-    1-20 steps, and has its own Poisson generator.  Returns the node
-    sequence of the neurons, the instance ranges and the synapse count."""
+    1-20 steps (drawn per synapse by the library), and has its own Poisson
+    generator.  Returns the node sequence of the neurons, the instance
+    ranges and the synapse count."""
     ngpu.ResetKernel()
     ngpu.SetVerbosityLevel(0)
     ngpu.SetRandomSeed(seed)
@@ -284,14 +334,12 @@ def build_ensemble(ngpu, instances, seed=6464, tol=1e-3, w_p=0.1):
         k_i = min(k - int(round(0.8 * k)), n - n_e)
         net = neuron[i0:i0 + n]
         exc, inh = neuron[i0:i0 + n_e], neuron[i0 + n_e:i0 + n]
-        d_e = (rng.integers(1, 21, n * k_e) * 0.1).astype(np.float32)
+        delay = {"distribution": "uniform", "low": 0.1, "high": 2.0,
+                 "step": 0.1}
         ngpu.Connect(exc, net, {"rule": "fixed_indegree", "indegree": k_e},
-                     {"weight": x["w_e"], "delay": {"array": d_e},
-                      "receptor": 0})
-        d_i = (rng.integers(1, 21, n * k_i) * 0.1).astype(np.float32)
+                     {"weight": x["w_e"], "delay": delay, "receptor": 0})
         ngpu.Connect(inh, net, {"rule": "fixed_indegree", "indegree": k_i},
-                     {"weight": x["w_i"], "delay": {"array": d_i},
-                      "receptor": 1})
+                     {"weight": x["w_i"], "delay": delay, "receptor": 1})
         ngpu.SetStatus(pg[j:j + 1], {"rate": x["rate"]})
         ngpu.Connect(pg[j:j + 1], net, {"rule": "all_to_all"},
                      {"weight": w_p, "delay": 0.1, "receptor": 0})

@@ -86,6 +86,32 @@ def test_single_neuron_fixed_drive(ngpu, I_e, tol):
         assert relW <= 1e-5
 
 
+@pytest.mark.parametrize("I_e", [700.0, 1000.0])
+def test_spiking_neuron_fp64_spike_times(ngpu, I_e):
+    """aeif_cond_beta above rheobase under constant current for 1 s at the
+    configurations' RK5 tolerance 1e-3: the GPU's spike train (FP32 state)
+    has the FP64 oracle's spike count, and every spike lies within one
+    time step of the FP64 oracle's."""
+    log("case fp64_spike_times I_e", I_e)
+    spec = {"groups": [{"model": "aeif_cond_beta", "n": 1, "n_port": 2,
+                        "params": {"I_e": I_e}, "rk5": {"rk5_tol": 1e-3}}]}
+    groups, _ = nb.build_engine(ngpu, spec)
+    ngpu.ActivateRecSpikeTimes(groups[0], 1000)
+    ngpu.Simulate(1000.0)
+    eng = np.array(ngpu.GetRecSpikeTimes(0), dtype=np.float64)
+    n64, _ = nb.build_oracle(spec, "f64")
+    n64.run(10000)
+    orc = nb.oracle_times_by_neuron(n64, 0.1, 0, 1)[0]
+    log("spikes gpu", eng.size, "fp64", orc.size)
+    assert eng.size == orc.size and eng.size >= 10
+    d = np.abs(eng - orc)
+    isi_g, isi_o = np.diff(eng), np.diff(orc)
+    log("max |dt| ms", d.max(), "mean ISI gpu", isi_g.mean(), "fp64",
+        isi_o.mean())
+    assert d.max() <= 0.1 + 1e-4
+    assert abs(isi_g.mean() - isi_o.mean()) <= 1e-3 * isi_o.mean()
+
+
 def test_spike_detector_delays(ngpu):
     """python/test/test_spike_detector.py: a spike recorded at row i with
     delay 10/20/30 ms appears at detector rows i+101/201/301 with heights
@@ -486,16 +512,6 @@ def test_overlapped_steps_then_graph_steps(ngpu):
     assert got == ref
 
 
-def _conn_table(ngpu, n):
-    ids = ngpu.GetConnections(ngpu.NodeSeq(0, n), ngpu.NodeSeq(0, n))
-    out = []
-    for c in ids:
-        st = ngpu.GetConnectionStatus(c)
-        out.append((st["source"], st["target"], st["port"], st["syn"],
-                    st["delay"], st["weight"]))
-    return out
-
-
 def _build_rules(ngpu, device):
     """Every rule the device generates, with ranges and node lists, scalar
     and per-connection weights and delays, both sampler paths (n >= 5k:
@@ -535,6 +551,18 @@ def _build_rules(ngpu, device):
     ngpu.Connect(big[0:20000], b[0:2], {"rule": "fixed_indegree",
                                        "indegree": 3500},
                  {"weight": 0.05, "delay": 0.2})
+    # weights and delays drawn by the library per connection
+    ngpu.Connect(a, b[0:300], {"rule": "fixed_indegree", "indegree": 60},
+                 {"weight": {"distribution": "normal", "mu": 1.0,
+                             "sigma": 0.3},
+                  "delay": {"distribution": "uniform", "low": 0.1,
+                            "high": 2.0, "step": 0.1}})
+    ngpu.Connect(a[0:200], a[200:400], {"rule": "all_to_all"},
+                 {"weight": {"distribution": "uniform", "low": 0.2,
+                             "high": 0.6},
+                  "delay": {"distribution": "normal_clipped", "mu": 0.5,
+                            "sigma": 0.25, "low": 0.1, "high": 1.25},
+                  "receptor": 1})
     ngpu.Simulate(1.0)
     return n_tot
 
@@ -545,13 +573,16 @@ def test_device_connect_matches_host(ngpu):
     target, port, synapse group, delay and weight, in the same order."""
     log("case device_connect")
     n = _build_rules(ngpu, 0)
-    host = _conn_table(ngpu, n)
+    host = ngpu.GetConnectionArrays()
     _build_rules(ngpu, 1)
-    dev = _conn_table(ngpu, n)
-    log("connections", len(host))
+    dev = ngpu.GetConnectionArrays()
+    log("connections", host["source"].size)
+    for k in host:
+        assert np.array_equal(dev[k], host[k]), k
+    host = list(zip(*[host[k] for k in ("source", "target", "port", "syn",
+                                        "delay", "weight")]))
     assert len(host) == 900 * 50 + 1200 * 40 + 500 * 30 + 60 * 20 + 5000 + \
-        30 * 40 + 70 + 2 * 3500
-    assert dev == host
+        30 * 40 + 70 + 2 * 3500 + 300 * 60 + 200 * 200
 
 
 def test_graph_steps_stdp_network(ngpu):

@@ -226,6 +226,76 @@ def test_weight_delay_distributions(ngpu):
     assert abs(w.mean() - 1.0) < 0.05
 
 
+def test_library_drawn_distributions(ngpu):
+    """Weights and delays drawn by the library per connection (the synapse
+    dictionary's distributions) follow the distribution: Kolmogorov-Smirnov
+    against scipy's normal, truncated normal and uniform CDFs, and a
+    chi-square over the 20 values of a stepped uniform delay."""
+    from scipy import stats
+    n1, n2 = 300000, 2000000
+
+    def one(conn, syn, src, tgt):
+        ngpu.ResetKernel()
+        ngpu.SetVerbosityLevel(0)
+        nodes = ngpu.Create("parrot_neuron", 3000)
+        ngpu.Connect(nodes[src[0]:src[1]], nodes[tgt[0]:tgt[1]], conn, syn)
+        return ngpu.GetConnectionArrays()
+
+    c1 = one({"rule": "fixed_indegree", "indegree": 100},
+             {"weight": {"distribution": "normal", "mu": 1.0, "sigma": 0.3},
+              "delay": {"distribution": "uniform", "low": 0.1, "high": 2.0,
+                        "step": 0.1}}, (0, 3000), (0, 3000))
+    assert c1["weight"].size == n1
+    p = stats.kstest(c1["weight"].astype(np.float64),
+                     stats.norm(1.0, 0.3).cdf).pvalue
+    steps = np.rint(c1["delay"] / np.float32(0.1)).astype(int)
+    h = np.bincount(steps - 1, minlength=20)
+    chi2 = np.sum((h - n1 / 20.0) ** 2) / (n1 / 20.0)
+    assert p > 1e-3 and h.size == 20 and chi2 < 60.0
+    c2 = one({"rule": "all_to_all"},
+             {"weight": {"distribution": "normal_clipped", "mu": 0.5,
+                         "sigma": 0.25, "low": 0.1, "high": 1.25},
+              "delay": {"distribution": "normal_clipped", "mu": 1.5,
+                        "sigma": 0.75, "low": 0.1, "high": 4.0}},
+             (0, 1000), (1000, 3000))
+    assert c2["weight"].size == n2
+    tn = stats.truncnorm((0.1 - 0.5) / 0.25, (1.25 - 0.5) / 0.25, 0.5, 0.25)
+    assert stats.kstest(c2["weight"].astype(np.float64), tn.cdf).pvalue > 1e-3
+    # delays: round(d / dt) slots of a truncated normal
+    td = stats.truncnorm((0.1 - 1.5) / 0.75, (4.0 - 1.5) / 0.75, 1.5, 0.75)
+    k = np.rint(c2["delay"] / np.float32(0.1)).astype(int)
+    edges = (np.arange(1, 42) - 0.5) * 0.1
+    pk = np.diff(td.cdf(np.clip(edges, 0.1, 4.0)))
+    hk = np.bincount(k - 1, minlength=40)[:40]
+    ok = pk * n2 > 50
+    chi2 = np.sum((hk[ok] - pk[ok] * n2) ** 2 / (pk[ok] * n2))
+    assert chi2 < ok.sum() + 8.0 * np.sqrt(2.0 * ok.sum())
+    c3 = one({"rule": "one_to_one"},
+             {"weight": {"distribution": "uniform", "low": -0.5, "high": 0.7},
+              "delay": 0.3}, (0, 500), (500, 1000))
+    w3 = c3["weight"].astype(np.float64)
+    assert w3.min() > -0.5 and w3.max() <= 0.7
+    assert stats.kstest(w3, stats.uniform(-0.5, 1.2).cdf).pvalue > 1e-3
+
+
+def test_distribution_errors(ngpu):
+    a = ngpu.Create("parrot_neuron", 100)
+    with pytest.raises(ValueError, match="time resolution"):
+        # unbounded normal delays: some fall below the resolution
+        ngpu.Connect(a, a, {"rule": "fixed_indegree", "indegree": 50},
+                     {"weight": 1.0, "delay": {"distribution": "normal",
+                                               "mu": 0.2, "sigma": 1.0}})
+        ngpu.GetConnectionArrays()
+    with pytest.raises(ValueError, match="probability"):
+        ngpu.Connect(a, a, {"rule": "all_to_all"},
+                     {"weight": {"distribution": "normal_clipped", "mu": 0.0,
+                                 "sigma": 0.1, "low": 5.0, "high": 6.0},
+                      "delay": 1.0})
+    with pytest.raises(ValueError, match="Unknown distribution"):
+        ngpu.Connect(a, a, {"rule": "all_to_all"},
+                     {"weight": {"distribution": "lognormal"}, "delay": 1.0})
+
+
 def test_connection_status_fields(ngpu):
     a = ngpu.Create("aeif_cond_beta", 2, 3)
     sg = ngpu.CreateSynGroup("stdp")
@@ -237,6 +307,52 @@ def test_connection_status_fields(ngpu):
     assert d["weight"] == 2.5
     # delay rounded to whole steps: round(0.75 / 0.1) = 8 steps
     assert abs(d["delay"] - 0.8) < 1e-6
+
+
+def test_bulk_connection_queries(ngpu):
+    """GetConnectionArrays returns, in one call, what GetConnections and
+    GetConnectionStatus give one connection at a time (same order); the
+    host path of GetConnectionCounts equals counts taken from it."""
+    rng = np.random.default_rng(3)
+    a = ngpu.Create("aeif_cond_beta", 60, 2)
+    b = ngpu.Create("iaf_psc_exp", 40)
+    sg = ngpu.CreateSynGroup("stdp")
+    ngpu.Connect(a, b, {"rule": "fixed_indegree", "indegree": 7},
+                 {"weight": {"array": rng.normal(1.0, 0.2, 280)
+                             .astype(np.float32)},
+                  "delay": {"array": rng.integers(1, 9, 280)
+                            .astype(np.float32) * 0.1}, "receptor": 1})
+    ngpu.Connect(b[0:10], a, {"rule": "fixed_outdegree", "outdegree": 5},
+                 {"weight": 0.5, "delay": 0.3, "synapse_group": sg})
+    ngpu.Connect(a[0:20], a[20:40], {"rule": "one_to_one"},
+                 {"weight": 0.25, "delay": 1.2, "receptor": 0})
+    ids = ngpu.GetConnections()
+    ref = [ngpu.GetConnectionStatus(c) for c in ids]
+    got = ngpu.GetConnectionArrays()
+    assert got["source"].size == len(ref) == 280 + 50 + 20
+    for k, d in enumerate(ref):
+        assert (got["source"][k], got["target"][k], got["port"][k],
+                got["syn"][k]) == (d["source"], d["target"], d["port"],
+                                   d["syn"])
+        assert got["delay"][k] == np.float32(d["delay"])
+        assert got["weight"][k] == np.float32(d["weight"])
+    sub = ngpu.GetConnectionArrays(source=a, target=b)
+    m = (got["source"] < 60) & (got["target"] >= 60)
+    assert np.array_equal(sub["target"], got["target"][m])
+    steps = np.rint(got["delay"] / np.float32(0.1)).astype(int)
+    for port in (-1, 0, 1):
+        m = got["port"] == port if port >= 0 else np.ones(got["port"].size,
+                                                         bool)
+        cnt = ngpu.GetConnectionCounts(a, port=port, n_delay=15)
+        assert np.array_equal(cnt["in_degree"],
+                              np.bincount(got["target"][m], minlength=100)
+                              [:60])
+        assert np.array_equal(cnt["out_degree"],
+                              np.bincount(got["source"][m], minlength=100)
+                              [:60])
+        own = m & (got["source"] < 60)
+        assert np.array_equal(cnt["delay_count"],
+                              np.bincount(steps[own] - 1, minlength=15))
 
 
 # ---------------------------------------------------------------------

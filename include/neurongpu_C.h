@@ -227,8 +227,8 @@ extern "C" {
    * node i_node + i as target / source, delay_count[d] = synapses with a
    * source in the range and a delay of d + 1 time steps (d < n_delay).
    * Only synapses on receptor `port` count (every port if port < 0).
-   * Poisson generators' direct connections are not synapses of the store
-   * and are not counted.  Any output may be NULL. */
+   * Connections from poisson_generator nodes count like any other, as in
+   * GetConnections.  Any output may be NULL. */
   int NeuronGPU_GetConnectionCounts(int i_node, int n_node, int port,
                                     int *in_degree, int *out_degree,
                                     long long *delay_count, int n_delay);
@@ -256,6 +256,7 @@ extern "C" {
   /* the proposer's output (log2 of the step in units of dt, before the
    * safety factor) for n feature vectors feat[n][6] */
   int NeuronGPU_ProposerEval(float *feat, int n, float *out);
+
 
 #ifdef __cplusplus
 }

@@ -467,15 +467,16 @@ __device__ __forceinline__ void aeif_store_state(const AeifArgs &A, int i,
                                                  const float (&y)[2 + 2 * P],
                                                  double x, float h,
                                                  float refr) {
-  A.x[i] = x;
-  A.h[i] = h;
-  A.refr[i] = refr;
-  A.V[i] = y[0];
-  A.w[i] = y[1];
+  const uint32_t u = (uint32_t)i;
+  at(A.x, u) = x;
+  at(A.h, u) = h;
+  at(A.refr, u) = refr;
+  at(A.V, u) = y[0];
+  at(A.w, u) = y[1];
 #pragma unroll
   for (int q = 0; q < P; q++) {
-    A.g[q][i] = y[2 + 2 * q];
-    A.g1[q][i] = y[3 + 2 * q];
+    at(A.g[q], u) = y[2 + 2 * q];
+    at(A.g1[q], u) = y[3 + 2 * q];
   }
 }
 
@@ -546,21 +547,22 @@ void aeif_pass1_kernel(AeifArgs A) {
     AeifLocal<P> p;
     aeif_load_params<P, UNI>(A, i, p);
     float y[N], k1[N], inv_scal[N];
-    y[0] = A.V[i];
-    y[1] = A.w[i];
+    const uint32_t u = (uint32_t)i;
+    y[0] = at(A.V, u);
+    y[1] = at(A.w, u);
 #pragma unroll
     for (int q = 0; q < P; q++) {
-      y[2 + 2 * q] = A.g[q][i];
-      y[3 + 2 * q] = A.g1[q][i];
+      y[2 + 2 * q] = at(A.g[q], u);
+      y[3 + 2 * q] = at(A.g1[q], u);
     }
     float in[P];
     gather_input<P>(A.fold, sn, node, in);
 #pragma unroll
     for (int q = 0; q < P; q++)
       y[3 + 2 * q] = __builtin_fmaf(in[q], p.g0[q], y[3 + 2 * q]);
-    double x = A.x[i];
-    float h = A.h[i];
-    float refr = A.refr[i];
+    double x = at(A.x, u);
+    float h = at(A.h, u);
+    float refr = at(A.refr, u);
     int n_att = 0;
     bool need_k1 = true;
     float h_begin = h, err_last = 1.0f, hh_used = h;
@@ -707,16 +709,17 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       n_att = catch_up ? 0 : (int)r.n_att;
       err_last = catch_up ? 1.0f : r.err_last;
       aeif_load_params<P, UNI>(A, i, p);
-      y[0] = A.V[i];
-      y[1] = A.w[i];
+      const uint32_t u = (uint32_t)i;
+      y[0] = at(A.V, u);
+      y[1] = at(A.w, u);
 #pragma unroll
       for (int q = 0; q < P; q++) {
-        y[2 + 2 * q] = A.g[q][i];
-        y[3 + 2 * q] = A.g1[q][i];
+        y[2 + 2 * q] = at(A.g[q], u);
+        y[3 + 2 * q] = at(A.g1[q], u);
       }
-      x = A.x[i];
-      h = A.h[i];
-      refr = A.refr[i];
+      x = at(A.x, u);
+      h = at(A.h, u);
+      refr = at(A.refr, u);
       if (catch_up) {
         // the whole step, as the first pass runs it: input fold, then
         // RK5 steps from the stored step size

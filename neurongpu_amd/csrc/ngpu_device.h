@@ -68,6 +68,17 @@ __device__ __forceinline__ float dev_logf(float x) {
   return x != x ? x : res;
 }
 
+// Element i of a device array, addressed by a 32-bit byte offset from the
+// array base (wave-uniform, a kernel argument): the global memory
+// instructions then take the base in scalar registers and the offset in
+// one vector register, instead of a 64-bit vector address per access.
+// Valid while i * sizeof(T) < 2^32 (node indices are below 2^26).
+template <class T>
+__device__ __forceinline__ T &at(T *base, uint32_t i) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) +
+                                i * (uint32_t)sizeof(T));
+}
+
 // err^(-1/4) and err^(-1/5) of the step-size controller.
 __device__ __forceinline__ float dev_pow_dec(float e) {
   return 1.0f / sqrtf(sqrtf(e));

@@ -128,9 +128,9 @@ __device__ __forceinline__ void gather_input(const FoldIn &f,
                                              float (&a)[P]) {
 #pragma unroll
   for (int p = 0; p < P; p++) {
-    float *ap = f.acc + (long long)p * f.n_total + node;
-    a[p] = *ap;
-    if (a[p] != 0.0f) *ap = 0.0f;
+    float &ap = at(f.acc + (long long)p * f.n_total, (uint32_t)node);
+    a[p] = ap;
+    if (a[p] != 0.0f) ap = 0.0f;
   }
   if (sn.pois_T < 0 || !f.dir.active) return;
   const DirectIn &d = f.dir;
@@ -167,9 +167,9 @@ __device__ __forceinline__ void gather_input(const FoldIn &f,
 // direct connections to that port in canonical order.
 __device__ __forceinline__ float gather_port(const FoldIn &f, long long pois_T,
                                              double pois_t, int node, int p) {
-  float *ap = f.acc + (long long)p * f.n_total + node;
-  float a = *ap;
-  if (a != 0.0f) *ap = 0.0f;
+  float &ap = at(f.acc + (long long)p * f.n_total, (uint32_t)node);
+  float a = ap;
+  if (a != 0.0f) ap = 0.0f;
   if (pois_T < 0 || !f.dir.active) return a;
   const DirectIn &d = f.dir;
   uint32_t Tlo = (uint32_t)pois_T, Thi = (uint32_t)((uint64_t)pois_T >> 32);

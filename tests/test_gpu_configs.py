@@ -83,8 +83,14 @@ def test_c1_reduced_n_bit_exact(ngpu):
     neuron, n_syn, v0 = wl.build_c1(ngpu, n=n)
     ngpu.ActivateSpikeCount(neuron)
     ngpu.Calibrate()
+    # every connection onto the neurons: the recurrent ones and one from the
+    # Poisson generator (node n) per neuron, as GetConnections lists them
+    c_all = ngpu.GetConnectionArrays(target=neuron)
     c = ngpu.GetConnectionArrays(source=neuron, target=neuron)
     assert c["source"].size == n_syn == n * 1000
+    assert c_all["source"].size == n_syn + n
+    gen = c_all["source"] == n
+    assert np.array_equal(np.sort(c_all["target"][gen]), np.arange(n))
     for port, lo, hi, k, w in ((0, 0, n_e, 800, 0.05), (1, n_e, n, 200,
                                                         0.25)):
         m = c["port"] == port
@@ -98,7 +104,9 @@ def test_c1_reduced_n_bit_exact(ngpu):
         assert steps.min() == 1 and steps.max() == 20
         # the device's counts agree with the store read back
         cnt = ngpu.GetConnectionCounts(neuron, port=port, n_delay=20)
-        assert np.array_equal(cnt["in_degree"], np.bincount(tgt, minlength=n))
+        ma = c_all["port"] == port
+        assert np.array_equal(cnt["in_degree"],
+                              np.bincount(c_all["target"][ma], minlength=n))
         assert np.array_equal(cnt["out_degree"],
                               np.bincount(src, minlength=n))
         assert np.array_equal(cnt["delay_count"],
@@ -122,7 +130,8 @@ def test_c1_reduced_n_bit_exact(ngpu):
 def test_c1_full_size_store(ngpu):
     """The 1e6-neuron / 1e9-synapse store as the benchmark builds it (device
     fixed_indegree rule): every neuron has exactly 800 synapses on port 0,
-    all from E neurons, and 200 on port 1, all from I neurons; the
+    all from E neurons (plus the one from the Poisson generator), and 200
+    on port 1, all from I neurons; the
     out-degrees of the sources are those of uniform sampling (chi-square
     and range); delays are uniform over the 20 slots (chi-square); the 200
     neurons of a sampled range have no multapse."""
@@ -133,7 +142,8 @@ def test_c1_full_size_store(ngpu):
     for port, lo, hi, k in ((0, 0, n_e, 800), (1, n_e, n, 200)):
         cnt = ngpu.GetConnectionCounts(neuron, port=port, n_delay=21)
         ind, outd, dc = cnt["in_degree"], cnt["out_degree"], cnt["delay_count"]
-        assert np.all(ind == k), (port, ind.min(), ind.max())
+        n_gen = 1 if port == 0 else 0   # the Poisson generator's synapse
+        assert np.all(ind == k + n_gen), (port, ind.min(), ind.max())
         # sources in range: no synapse on this port from the other class
         assert outd[:lo].sum() == 0 and outd[hi:].sum() == 0
         assert outd[lo:hi].sum() == n * k
@@ -154,7 +164,7 @@ def test_c1_full_size_store(ngpu):
         log("port", port, "delay chi2 (19 dof)", chi2_d)
         assert chi2_d < 60.0
     t0 = 123456
-    c = ngpu.GetConnectionArrays(target=ngpu.NodeSeq(t0, 200))
+    c = ngpu.GetConnectionArrays(source=neuron, target=ngpu.NodeSeq(t0, 200))
     assert c["source"].size == 200 * 1000
     key = c["target"].astype(np.int64) * n + c["source"]
     assert np.unique(key).size == key.size

@@ -488,6 +488,10 @@ class NeuronGPU {
   // generator, 4 delivery, 5 post-synaptic STDP, 6 record)
   int SetKernelTiming(int on);
   int GetKernelTimes(double *ms8, long long *cnt8);
+  // no reference counterpart: one value per node for a scalar parameter or
+  // variable of the nodes i_node .. i_node+n-1
+  int SetNeuronScalArray(int i_node, int n, std::string name,
+                         const float *vals);
   // no reference counterpart: bulk connection queries (canonical order of
   // GetConnections; n_source / n_target <= 0 mean every node) and synapse
   // counts per node and delay slot

@@ -210,6 +210,12 @@ extern "C" {
   int NeuronGPU_SetKernelTiming(int on);
   int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8);
 
+  /* One value per node (vals[0 .. n_node-1]) for the scalar parameter or
+   * variable `name` of nodes i_node .. i_node + n_node - 1 (one node
+   * group), where the reference's Python makes one call per node. */
+  int NeuronGPU_SetNeuronScalArray(int i_node, int n_node, char *name,
+                                   float *vals);
+
   /* Bulk form of GetConnections + GetConnectionStatus: the connections
    * with source in [i_source, i_source + n_source) and target in
    * [i_target, i_target + n_target) (n <= 0: every node), in the canonical

@@ -464,6 +464,9 @@ int NeuronGPU_SetKernelTiming(int on)
     RET(int, 0, ret = instance->SetKernelTiming(on))
 int NeuronGPU_GetKernelTimes(double *ms8, long long *cnt8)
     RET(int, 0, ret = instance->GetKernelTimes(ms8, cnt8))
+int NeuronGPU_SetNeuronScalArray(int i_node, int n_node, char *name,
+                                 float *vals)
+    RET(int, 0, ret = instance->SetNeuronScalArray(i_node, n_node, name, vals))
 long long NeuronGPU_GetConnectionArrays(int i_source, int n_source,
                                         int i_target, int n_target,
                                         long long n_max, int *source,

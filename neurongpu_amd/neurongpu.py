@@ -200,6 +200,7 @@ _SIG = {
     "ProposerSetWeights": (c_int, (c_void_p,)),
     "ProposerUse": (c_int, (c_int,)),
     "ProposerEval": (c_int, (c_void_p, c_int, c_void_p)),
+    "SetNeuronScalArray": (c_int, (c_int, c_int, c_char_p, c_void_p)),
     "GetConnectionArrays": (ctypes.c_longlong,
                             (c_int, c_int, c_int, c_int, ctypes.c_longlong,
                              c_void_p, c_void_p, c_void_p, c_void_p,
@@ -797,9 +798,11 @@ def SetNeuronStatus(nodes, var_name, val):
         if isinstance(nodes, NodeSeq):
             i0 = nodes.i0
             if IsNeuronScalParam(i0, var_name) or IsNeuronScalVar(i0, var_name):
-                # one value per node, set as a block (same result as the
+                # one value per node in one call (same result as the
                 # reference's per-node loop, pythonlib/neurongpu.py:1146-1151)
-                _set_per_node(nodes, var_name, arr)
+                a = np.ascontiguousarray(arr, dtype=np.float32)
+                _call("SetNeuronScalArray", i0, nodes.n, _b(var_name),
+                      a.ctypes.data_as(c_void_p))
                 return
         for i in range(n):
             SetNeuronStatus([nodes[i]], var_name, float(arr[i]))
@@ -836,22 +839,6 @@ def SetNeuronStatus(nodes, var_name, val):
             SetNeuronPtArrayVar(nodes, var_name, val)
         else:
             raise ValueError("Unknown neuron variable or parameter")
-
-
-def _set_per_node(nodes, var_name, arr):
-    # distinct value per node through single-node calls grouped by value run
-    vals = np.asarray(arr, dtype=np.float32)
-    scal_param = IsNeuronScalParam(nodes.i0, var_name)
-    fn = "SetNeuronPtScalParam" if scal_param else "SetNeuronPtScalVar"
-    order = np.argsort(vals, kind="stable")
-    sv = vals[order]
-    cuts = np.flatnonzero(np.diff(sv)) + 1
-    starts = np.concatenate(([0], cuts))
-    ends = np.concatenate((cuts, [len(sv)]))
-    for s, e in zip(starts, ends):
-        idx = (nodes.i0 + order[s:e]).astype(np.int32)
-        _call(fn, idx.ctypes.data_as(c_void_p), int(idx.size), _b(var_name),
-              float(sv[s]))
 
 
 def SetStatus(gen_object, params, val=None):

@@ -1651,18 +1651,24 @@ template <int P>
 void launch_aeif_pass2_p(const AeifArgs &a, const DeliveryArgs *d,
                          int d_grid, hipStream_t s) {
   int grid2 = a.pass2_grid > 0 ? a.pass2_grid : 256;
-  // the catch-up of neurons an overlapped step left (without the proposer)
+  // the proposer's inference exists for groups of up to kPropMaxPort ports
+  constexpr bool kProp = P <= kPropMaxPort;
+  // the catch-up of neurons an overlapped step left
   if constexpr (P <= kOverlapMaxPort) {
     if (a.skip_prev || a.mask_write) {
-      if (a.uniform)
+      if (a.prop_pk && kProp) {
+        if (a.uniform)
+          launch_pass2_v<P, true, kProp, true>(a, d, grid2, d_grid, s);
+        else
+          launch_pass2_v<P, false, kProp, true>(a, d, grid2, d_grid, s);
+      } else if (a.uniform) {
         launch_pass2_v<P, true, false, true>(a, d, grid2, d_grid, s);
-      else
+      } else {
         launch_pass2_v<P, false, false, true>(a, d, grid2, d_grid, s);
+      }
       return;
     }
   }
-  // the proposer's inference exists for groups of up to kPropMaxPort ports
-  constexpr bool kProp = P <= kPropMaxPort;
   if (a.prop_pk && kProp) {
     if (a.uniform)
       launch_pass2_v<P, true, kProp, false>(a, d, grid2, d_grid, s);

@@ -514,6 +514,7 @@ class NeuronGPU {
   int ProposerSetWeights(const float *w);
   int ProposerUse(int on);
   int ProposerEval(const float *feat, int n, float *out);
+  int ProposerEvalNoAlloc(const float *feat, int n, float *out);
 };
 
 #endif

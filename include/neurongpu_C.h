@@ -263,6 +263,15 @@ extern "C" {
    * safety factor) for n feature vectors feat[n][6] */
   int NeuronGPU_ProposerEval(float *feat, int n, float *out);
 
+  /* No counterpart in the reference.  NeuronGPU_ProposerEval for callers
+   * that cannot allocate device memory (code under HIP graph capture, tight
+   * serving loops, harnesses that forbid hipMalloc): the same kernel and
+   * results, with the feature vectors and outputs staged through a
+   * file-scope device array instead of buffers allocated per call.  At most
+   * 65536 feature vectors per call (larger n is an error); the weights must
+   * exist already (ProposerSetWeights, ProposerTrain or ProposerEval).
+   * Calls that overlap on one GPU share the staging array and race. */
+  int NeuronGPU_ProposerEvalNoAlloc(float *feat, int n, float *out);
 
 #ifdef __cplusplus
 }

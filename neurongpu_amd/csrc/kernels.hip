@@ -1692,6 +1692,49 @@ void LaunchPropEval(const unsigned char *pk, const float *feat, int n,
                        (hipStream_t)stream, pk, feat, n, out);
 }
 
+// Staging of the allocation-free inference: the feature vectors and the
+// outputs of one call, in a file-scope device array (one copy per GPU, held
+// until the process ends).  Host code never uses the array's name as an
+// address: each device's copy is found once with hipGetSymbolAddress.
+namespace {
+constexpr size_t kPropScratchFloats = (size_t)kPropEvalNoAllocMax * (kPropIn + 1);
+__device__ __align__(256) float prop_scratch[kPropScratchFloats];
+constexpr int kMaxDevices = 64;
+float *prop_scratch_dev[kMaxDevices];   // per device, null until looked up
+}  // namespace
+
+const char *PropEvalNoAlloc(const unsigned char *pk, const float *feat,
+                            int n, float *out, void *stream) {
+  if (n < 0) return "negative number of feature vectors";
+  if (n > kPropEvalNoAllocMax)
+    return "more feature vectors than the allocation-free inference holds "
+           "(kPropEvalNoAllocMax)";
+  if (n == 0) return nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return "no usable HIP device";
+  if (!prop_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(prop_scratch)) != hipSuccess ||
+        !p)
+      return "hipGetSymbolAddress of the inference staging failed";
+    prop_scratch_dev[dev] = static_cast<float *>(p);
+  }
+  float *f_dev = prop_scratch_dev[dev];
+  float *o_dev = f_dev + (size_t)kPropEvalNoAllocMax * kPropIn;
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(f_dev, feat, (size_t)n * kPropIn * sizeof(float),
+                     hipMemcpyHostToDevice, s) != hipSuccess)
+    return "copy of the feature vectors failed";
+  LaunchPropEval(pk, f_dev, n, o_dev, stream);
+  if (hipGetLastError() != hipSuccess) return "inference launch failed";
+  if (hipMemcpyAsync(out, o_dev, (size_t)n * sizeof(float),
+                     hipMemcpyDeviceToHost, s) != hipSuccess)
+    return "copy of the outputs failed";
+  if (hipStreamSynchronize(s) != hipSuccess) return "inference failed";
+  return nullptr;
+}
+
 void LaunchAeifPass2(const AeifArgs &a, int n_port, const DeliveryArgs *d,
                      int d_grid, void *stream) {
   hipStream_t s = (hipStream_t)stream;

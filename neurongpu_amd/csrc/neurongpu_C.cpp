@@ -506,4 +506,6 @@ int NeuronGPU_ProposerUse(int on)
     RET(int, 0, ret = instance->ProposerUse(on))
 int NeuronGPU_ProposerEval(float *feat, int n, float *out)
     RET(int, 0, ret = instance->ProposerEval(feat, n, out))
+int NeuronGPU_ProposerEvalNoAlloc(float *feat, int n, float *out)
+    RET(int, 0, ret = instance->ProposerEvalNoAlloc(feat, n, out))
 }

@@ -429,6 +429,13 @@ void LaunchAeif(const AeifArgs &a, int n_port, void *stream);
 // learned step-size proposer on n feature vectors ([n][6]) -> out[n]
 void LaunchPropEval(const unsigned char *pk, const float *feat, int n,
                     float *out, void *stream);
+// the same for host arrays feat / out, staged through a file-scope device
+// array of kPropEvalNoAllocMax vectors instead of allocated buffers (no
+// hipMalloc / hipFree); synchronous.  Calls that overlap on one GPU share
+// that array and race.  Returns an error message, or nullptr.
+constexpr int kPropEvalNoAllocMax = 65536;
+const char *PropEvalNoAlloc(const unsigned char *pk, const float *feat,
+                            int n, float *out, void *stream);
 // second pass of the two-pass update; with d != nullptr the delivery of
 // the step (non-plastic) runs in the same launch on d_grid more blocks
 void LaunchAeifPass2(const AeifArgs &a, int n_port, const DeliveryArgs *d,

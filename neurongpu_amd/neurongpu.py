@@ -200,6 +200,7 @@ _SIG = {
     "ProposerSetWeights": (c_int, (c_void_p,)),
     "ProposerUse": (c_int, (c_int,)),
     "ProposerEval": (c_int, (c_void_p, c_int, c_void_p)),
+    "ProposerEvalNoAlloc": (c_int, (c_void_p, c_int, c_void_p)),
     "SetNeuronScalArray": (c_int, (c_int, c_int, c_char_p, c_void_p)),
     "GetConnectionArrays": (ctypes.c_longlong,
                             (c_int, c_int, c_int, c_int, ctypes.c_longlong,
@@ -1570,6 +1571,16 @@ def ProposerEval(features):
     f = np.ascontiguousarray(features, dtype=np.float32)
     out = np.zeros(f.shape[0], dtype=np.float32)
     _call("ProposerEval", f.ctypes.data_as(c_void_p), f.shape[0],
+          out.ctypes.data_as(c_void_p))
+    return out
+
+
+def ProposerEvalNoAlloc(features):
+    """ProposerEval without device allocation (at most 65536 vectors; the
+    proposer's weights must exist already)"""
+    f = np.ascontiguousarray(features, dtype=np.float32)
+    out = np.zeros(f.shape[0], dtype=np.float32)
+    _call("ProposerEvalNoAlloc", f.ctypes.data_as(c_void_p), f.shape[0],
           out.ctypes.data_as(c_void_p))
     return out
 

@@ -25,6 +25,21 @@ const char *BuildReverseIndex(const unsigned long long *rec,
                               uint32_t **rev_idx_out,
                               unsigned long long *n_rev_out, void *stream);
 
+// Tile-major copy of the canonical store for the tiled delivery: trec
+// (S records) holds the records grouped by accumulator tile
+// ((port * n_total + target) >> tile_shift), canonical order inside a tile;
+// off ((N*D + 1) * T entries) gets off[(s*D + d)*T + t] = position in trec
+// of the first record of source s, delay slot d in tile t (row N*D: the
+// tile ends).  Requires S < 2^31 and T*N*D < 2^31.  Returns null on
+// success or a static error message.
+const char *BuildTiledStore(const unsigned long long *rec,
+                            const unsigned long long *base,
+                            const uint32_t *seg, unsigned long long S,
+                            uint32_t N, uint32_t D, uint32_t T,
+                            int tile_shift, long long n_total,
+                            unsigned long long *trec, uint32_t *off,
+                            void *stream);
+
 // Synapse counts of the canonical store (device pointers): for nodes lo ..
 // hi-1, in_deg[t - lo] += synapses with target t, out_deg[s - lo] +=
 // synapses with source s, delay_cnt[d] += synapses with source in the range

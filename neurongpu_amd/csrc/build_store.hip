@@ -106,6 +106,62 @@ __global__ __launch_bounds__(256) void count_synapses(
   }
 }
 
+// Tiled store: the accumulator tile of every canonical record, with its
+// canonical index as the sort value.
+__global__ void tile_keys(const unsigned long long *rec, unsigned long long S,
+                          long long n_total, int tile_shift, uint16_t *keys,
+                          uint32_t *vals) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  const uint32_t tp = (uint32_t)rec[i];
+  const long long a = (long long)(tp >> kPortShift) * n_total + (tp & kTargetMask);
+  keys[i] = (uint16_t)(a >> tile_shift);
+  vals[i] = (uint32_t)i;
+}
+
+__global__ void gather_u64(const uint32_t *idx, const unsigned long long *src,
+                           unsigned long long S, unsigned long long *dst) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < S) dst[i] = src[idx[i]];
+}
+
+// Records per (tile, source, delay slot), tile-major: one wavefront per
+// source walks its delay segments.
+__global__ __launch_bounds__(256) void tile_counts(
+    const unsigned long long *rec, const unsigned long long *base,
+    const uint32_t *seg, uint32_t N, uint32_t D, long long n_total,
+    int tile_shift, uint32_t *cnt) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n_wave = (gridDim.x * blockDim.x) >> 6;
+  const unsigned long long nd = (unsigned long long)N * D;
+  for (uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < N;
+       s += n_wave) {
+    const unsigned long long *r = rec + base[s];
+    const uint32_t *sg = seg + (unsigned long long)s * (D + 1);
+    for (uint32_t d = 0; d < D; d++) {
+      const uint32_t j1 = sg[d + 1];
+      for (uint32_t j = sg[d] + lane; j < j1; j += 64u) {
+        const uint32_t tp = (uint32_t)r[j];
+        const long long a =
+            (long long)(tp >> kPortShift) * n_total + (tp & kTargetMask);
+        const unsigned long long t = (unsigned long long)(a >> tile_shift);
+        atomicAdd(cnt + t * nd + (unsigned long long)s * D + d, 1u);
+      }
+    }
+  }
+}
+
+// off[(sd)*T + t] = start[t*ND + sd] for sd in [0, ND]
+__global__ void tile_transpose(const uint32_t *start, unsigned long long nd,
+                               uint32_t T, uint32_t *off) {
+  const unsigned long long n = (nd + 1) * T;
+  for (unsigned long long k = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+       k < n; k += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long sd = k / T, t = k - sd * T;
+    off[k] = start[t * nd + sd];
+  }
+}
+
 template <class T>
 struct DTmp {
   T *p = nullptr;
@@ -224,6 +280,68 @@ const char *BuildReverseIndex(const unsigned long long *rec,
   if (hipStreamSynchronize(stream) != hipSuccess) return "reverse index build failed";
   *rev_idx_out = idx;
   *n_rev_out = n_rev;
+  return nullptr;
+}
+
+const char *BuildTiledStore(const unsigned long long *rec,
+                            const unsigned long long *base,
+                            const uint32_t *seg, unsigned long long S,
+                            uint32_t N, uint32_t D, uint32_t T,
+                            int tile_shift, long long n_total,
+                            unsigned long long *trec, uint32_t *off,
+                            void *stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  const unsigned long long nd = (unsigned long long)N * D;
+  if (S >= 0x7FFFFFFFull || nd * T >= 0x7FFFFFFFull || T > 0xFFFFu)
+    return "synapse store too large for the tiled delivery";
+  if (S > 0) {
+    DTmp<uint16_t> k1(S), k2(S);
+    DTmp<uint32_t> v1(S), v2(S);
+    if (!k1.p || !k2.p || !v1.p || !v2.p)
+      return "out of device memory building the tiled store";
+    const unsigned int grid = (unsigned int)((S + 255) / 256);
+    hipLaunchKernelGGL(tile_keys, dim3(grid), dim3(256), 0, stream, rec, S,
+                       n_total, tile_shift, k1.p, v1.p);
+    hipcub::DoubleBuffer<uint16_t> kb(k1.p, k2.p);
+    hipcub::DoubleBuffer<uint32_t> vb(v1.p, v2.p);
+    const int end_bit = bits_for(T > 1 ? T - 1 : 1);
+    size_t sort_bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, kb, vb,
+                                             (int)S, 0, end_bit, stream);
+    DTmp<unsigned char> tmp(sort_bytes);
+    if (!tmp.p) return "out of device memory building the tiled store";
+    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, sort_bytes, kb, vb, (int)S,
+                                           0, end_bit, stream) != hipSuccess)
+      return "radix sort failed";
+    hipLaunchKernelGGL(gather_u64, dim3(grid), dim3(256), 0, stream,
+                       vb.Current(), rec, S, trec);
+    // the sort buffers are freed at the end of this scope
+    if (hipStreamSynchronize(stream) != hipSuccess)
+      return "tiled store build failed";
+  }
+  const unsigned long long nc = nd * T + 1;
+  DTmp<uint32_t> cnt(nc), start(nc);
+  if (!cnt.p || !start.p) return "out of device memory building the tiled store";
+  if (hipMemsetAsync(cnt.p, 0, nc * sizeof(uint32_t), stream) != hipSuccess)
+    return "hipMemsetAsync failed";
+  if (N > 0 && D > 0) {
+    unsigned int g = (N + 3) / 4;
+    if (g > 65536u) g = 65536u;
+    hipLaunchKernelGGL(tile_counts, dim3(g), dim3(256), 0, stream, rec, base,
+                       seg, N, D, n_total, tile_shift, cnt.p);
+  }
+  size_t scan_bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt.p, start.p,
+                                         (int)nc, stream);
+  DTmp<unsigned char> stmp(scan_bytes);
+  if (!stmp.p) return "out of device memory building the tiled store";
+  if (hipcub::DeviceScan::ExclusiveSum(stmp.p, scan_bytes, cnt.p, start.p,
+                                       (int)nc, stream) != hipSuccess)
+    return "scan failed";
+  hipLaunchKernelGGL(tile_transpose, dim3(4096), dim3(256), 0, stream,
+                     start.p, nd, T, off);
+  if (hipStreamSynchronize(stream) != hipSuccess)
+    return "tiled store build failed";
   return nullptr;
 }
 

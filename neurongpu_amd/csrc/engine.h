@@ -328,7 +328,7 @@ class Engine {
     hipGraphExec_t exec = nullptr;
     int steps = 0;
   };
-  StepGraph graphs[3];  // atomic, binned, atomic with the quiet grid
+  StepGraph graphs[4];  // atomic, binned, atomic with the quiet grid, tiled
   DBuf<StepClock> d_clock;
   bool clock_mode = false;    // launches being captured take d_clock
   int clock_slot = 0;         // clock of the step being captured
@@ -345,6 +345,15 @@ class Engine {
   DBuf<float> d_bin_part;
   // sets up the binned-delivery buffers; false if the network cannot use it
   bool PrepareBins(BinArgs &b);
+  // tiled delivery: the tile-major store copy, built on first use (false
+  // if the network cannot use it); the blocks per tile, 0: automatic
+  DBuf<unsigned long long> d_trec;
+  DBuf<uint32_t> d_toff;
+  DBuf<float> d_tpart;
+  bool tiled_built = false, tiled_failed = false;
+  int tiled_T = 0;
+  int delivery_tile_split = 0;
+  bool PrepareTiled(TileArgs &t);
   int max_port = 1;
   int R = 0;
   long long ring_cap = 0;

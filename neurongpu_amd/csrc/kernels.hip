@@ -500,7 +500,9 @@ __device__ __forceinline__ long long aeif_gs(const AeifArgs &A) {
 // fits in 80 VGPRs, which allows 6 wavefronts per SIMD instead of 5 (7 %
 // faster on c1, measured); for more ports or per-neuron parameters the same
 // limit would spill to scratch.
-template <int P, bool UNI, int W, bool OV>
+// B1: the attempt budget is 1 (the default), known at compile time, so the
+// attempt loop is straight-line code.
+template <int P, bool UNI, int W, bool OV, bool B1 = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 void aeif_pass1_kernel(AeifArgs A) {
   constexpr int N = 2 + 2 * P;
@@ -567,7 +569,7 @@ void aeif_pass1_kernel(AeifArgs A) {
     bool need_k1 = true;
     float h_begin = h, err_last = 1.0f, hh_used = h;
     for (;;) {
-      if (n_att >= A.budget) break;
+      if (n_att >= (B1 ? 1 : A.budget)) break;
       if (need_k1) {
         h_begin = h;
         aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
@@ -1024,7 +1026,9 @@ constexpr int kMaxDelaySlots = 4096;  // checked on the host at calibration
 // delay) pairs with delay slot < d due this step (d = 0..D), from the ring
 // counts of slots gs-1-d.  Virtual block 0 also clears the ring counter of
 // the slot the next step fills (slot gs+1 is never read in this step).
-// Returns the number of pairs.
+// Returns the number of pairs.  BS: threads per block; s_part holds BS
+// entries.
+template <int BS = kBlock>
 __device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
                                                       uint32_t vblock,
                                                       uint32_t *s_pre,
@@ -1053,7 +1057,7 @@ __device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
     return s_pre[D];
   }
   // per-thread partial sums of the due spike counts over d
-  const int per = (D + kBlock - 1) / kBlock;
+  const int per = (D + BS - 1) / BS;
   uint32_t run = 0;
   for (int k = 0; k < per; k++) {
     int d = tid * per + k;
@@ -1067,7 +1071,7 @@ __device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
   s_part[tid] = run;
   __syncthreads();
   // inclusive scan of the block partials (Hillis-Steele in LDS)
-  for (int off = 1; off < kBlock; off <<= 1) {
+  for (int off = 1; off < BS; off <<= 1) {
     uint32_t v = (tid >= off) ? s_part[tid - off] : 0u;
     __syncthreads();
     s_part[tid] += v;
@@ -1083,7 +1087,7 @@ __device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
         if (s >= 0) acc += A.ring_n[s % A.R];
       }
     }
-    if (tid == kBlock - 1) s_pre[D] = s_part[kBlock - 1];
+    if (tid == BS - 1) s_pre[D] = s_part[BS - 1];
   }
   __syncthreads();
   if (vblock == 0 && tid == 0) A.ring_n[(A.gs + 1) % A.R] = 0u;
@@ -1103,14 +1107,32 @@ __device__ __forceinline__ uint32_t delivery_prologue(const DeliveryArgs &A,
 // same blocks.  Returns the wave's event count.
 constexpr uint32_t kLongSegment = 32;
 
-template <int U, class F>
+// Synapse segment of (source, delay slot) in the canonical store: its
+// records rec[start .. start + cnt).
+struct SegLoc {
+  __device__ __forceinline__ void operator()(const DeliveryArgs &A,
+                                             uint32_t src, int d,
+                                             uint32_t &cnt,
+                                             unsigned long long &start) const {
+    const uint32_t *sg = A.seg + (size_t)src * (A.D + 1) + d;
+    const uint32_t o0 = sg[0], o1 = sg[1];
+    cnt = o1 - o0;
+    start = A.base[src] + o0;
+  }
+};
+
+// BS: threads per block; Loc(src, d, cnt, start) locates a pair's records
+// in `recs`.
+template <int U, int BS = kBlock, class F, class Loc = SegLoc>
 __device__ __forceinline__ unsigned long long deliver_events(
     const DeliveryArgs &A, uint32_t vblock, uint32_t vgrid,
-    const uint32_t *s_pre, uint32_t total, F &&f) {
+    const uint32_t *s_pre, uint32_t total, F &&f,
+    const unsigned long long *recs = nullptr, Loc loc = Loc{}) {
   const int D = A.D;
+  if (!recs) recs = A.rec;
   const uint32_t lane = lane_id();
-  const uint32_t wave = (uint32_t)(vblock * (kBlock / 64) + threadIdx.x / 64);
-  const uint32_t n_waves = vgrid * (kBlock / 64);
+  const uint32_t wave = (uint32_t)(vblock * (BS / 64) + threadIdx.x / 64);
+  const uint32_t n_waves = vgrid * (BS / 64);
   uint32_t pt = (uint32_t)A.tile_max;
   while (pt > 1 && (total + pt - 1) / pt < n_waves) pt >>= 1;
   const uint32_t n_tiles = (total + pt - 1) / pt;
@@ -1133,10 +1155,7 @@ __device__ __forceinline__ unsigned long long deliver_events(
       uint32_t src = A.ring_src[e];
       height = A.ring_h[e];
       mult = A.ring_cnt[e];
-      const uint32_t *sg = A.seg + (size_t)src * (D + 1) + d;
-      uint32_t o0 = sg[0], o1 = sg[1];
-      cnt = o1 - o0;
-      start = A.base[src] + o0;
+      loc(A, src, d, cnt, start);
     }
     // exclusive scan of cnt over the wavefront
     uint32_t incl = cnt;
@@ -1173,7 +1192,7 @@ __device__ __forceinline__ unsigned long long deliver_events(
             uint32_t e = o + 64u * u + lane;
             ok[u] = e < c;
             syn[u] = st + e;
-            r[u] = ok[u] ? A.rec[syn[u]] : 0ull;
+            r[u] = ok[u] ? recs[syn[u]] : 0ull;
           }
 #pragma unroll
           for (int u = 0; u < U; u++) f(ok[u], syn[u], r[u], h, mu);
@@ -1205,7 +1224,7 @@ __device__ __forceinline__ unsigned long long deliver_events(
         mj[u] = __shfl(mult, j);
         ok[u] = e < E;
         syn[u] = (((unsigned long long)shi << 32) | slo) + (e - pj);
-        r[u] = ok[u] ? A.rec[syn[u]] : 0ull;
+        r[u] = ok[u] ? recs[syn[u]] : 0ull;
       }
 #pragma unroll
       for (int u = 0; u < U; u++) f(ok[u], syn[u], r[u], hj[u], mj[u]);
@@ -1511,6 +1530,86 @@ __global__ __launch_bounds__(kBlock) void bin_sum_kernel(
 }
 
 // ---------------------------------------------------------------------
+// Tiled delivery (non-plastic).  Block b owns accumulator tile b / G in
+// LDS and takes share b % G of the step's (spike, delay) pairs; for each
+// pair it reads only the records of its tile (a contiguous group of the
+// tile-major store), so every record is read once per step by one block.
+// The same pair enumeration as the other delivery kernels, over the G
+// blocks of the tile.
+// ---------------------------------------------------------------------
+struct TileLoc {
+  const uint32_t *off;
+  uint32_t T, t;
+  __device__ __forceinline__ void operator()(const DeliveryArgs &A,
+                                             uint32_t src, int d,
+                                             uint32_t &cnt,
+                                             unsigned long long &start) const {
+    const size_t k = ((size_t)src * (size_t)A.D + (size_t)d) * T + t;
+    const uint32_t o0 = off[k], o1 = off[k + T];
+    cnt = o1 - o0;
+    start = o0;
+  }
+};
+
+__global__ __launch_bounds__(kTiledBlock) void tiled_delivery_kernel(
+    DeliveryArgs A0, TileArgs B) {
+  const DeliveryArgs A = clocked(A0);
+  __shared__ float s_acc[kTileLen];
+  __shared__ uint32_t s_part[kTiledBlock];
+  uint32_t *s_pre = s_dyn;
+  const uint32_t t = blockIdx.x / (uint32_t)B.G;
+  const uint32_t k = blockIdx.x - t * (uint32_t)B.G;
+  for (int i = threadIdx.x; i < kTileLen; i += kTiledBlock) s_acc[i] = 0.0f;
+  // (the prologue's barriers order the clearing before the first add)
+  const uint32_t total =
+      delivery_prologue<kTiledBlock>(A, blockIdx.x, s_pre, s_part);
+  const long long a0 = (long long)t << kTileShift;
+  const unsigned long long n_events = deliver_events<2, kTiledBlock>(
+      A, k, (uint32_t)B.G, s_pre, total,
+      [&](bool ok, unsigned long long, unsigned long long r, float hj,
+          uint32_t) {
+        if (!ok) return;
+        const uint32_t tp = (uint32_t)r;
+        const float w = __uint_as_float((uint32_t)(r >> 32));
+        const long long idx = (long long)(tp >> kPortShift) * A.n_total +
+                              (tp & kTargetMask);
+        lds_add_f32(&s_acc[idx - a0], hj * w);
+      },
+      B.trec, TileLoc{B.off, (uint32_t)B.T, t});
+  __syncthreads();
+  const long long lim_ll = B.acc_n - a0;
+  const int lim = lim_ll < kTileLen ? (int)lim_ll : kTileLen;
+  if (B.G == 1) {
+    for (int i = threadIdx.x; i < lim; i += kTiledBlock) {
+      const float x = s_acc[i];
+      if (x != 0.0f) A.acc[a0 + i] += x;
+    }
+  } else {
+    float *pp = B.part + (size_t)blockIdx.x * kTileLen;
+    for (int i = threadIdx.x; i < kTileLen; i += kTiledBlock) pp[i] = s_acc[i];
+  }
+  if (lane_id() == 0 && n_events)
+    atomicAdd(A.events + (blockIdx.x & (kCounterSlots - 1)), n_events);
+}
+
+// Sum of the G partial tiles of every accumulator entry (G > 1); the last
+// launch of the step (writes the next step's clock when captured).
+__global__ __launch_bounds__(kBlock) void tiled_sum_kernel(
+    TileArgs B, float *acc, const StepClock *clock, StepClock *clock_next) {
+  clock_step(clock, clock_next);
+  if (B.G <= 1) return;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+       i < B.acc_n; i += (long long)gridDim.x * kBlock) {
+    const long long t = i >> kTileShift;
+    const float *pp = B.part + (size_t)t * B.G * kTileLen +
+                      (size_t)(i & (kTileLen - 1));
+    float sum = 0.0f;
+    for (int k = 0; k < B.G; k++) sum += pp[(size_t)k * kTileLen];
+    if (sum != 0.0f) acc[i] += sum;
+  }
+}
+
+// ---------------------------------------------------------------------
 // Post-synaptic STDP (src/rev_spike.cu:41-75): every node whose spike
 // reaches its dendrites this step updates its plastic incoming synapses.
 // One wavefront per candidate node; lanes stride over its inputs.
@@ -1613,9 +1712,16 @@ void launch_aeif_p(const AeifArgs &a, hipStream_t s) {
   if (a.do_step && a.strag) {
     // the overlap bookkeeping is compiled in only where it is used (groups
     // of up to kOverlapMaxPort ports)
+    // the default budget of one attempt, compiled in for uniform
+    // parameters and up to two ports
+    const bool b1 = a.uniform && a.budget == 1 && P <= 2;
     if constexpr (P <= kOverlapMaxPort) {
       if (a.skip_prev || a.mask_write) {
-        if (a.uniform)
+        if (b1)
+          hipLaunchKernelGGL(
+              (aeif_pass1_kernel<P, true, (P <= 2 ? 6 : 1), true, true>),
+              dim3(grid), dim3(kBlock), 0, s, a);
+        else if (a.uniform)
           hipLaunchKernelGGL((aeif_pass1_kernel<P, true, (P <= 2 ? 6 : 1), true>),
                              dim3(grid), dim3(kBlock), 0, s, a);
         else
@@ -1624,7 +1730,11 @@ void launch_aeif_p(const AeifArgs &a, hipStream_t s) {
         return;
       }
     }
-    if (a.uniform)
+    if (b1)
+      hipLaunchKernelGGL(
+          (aeif_pass1_kernel<P, true, (P <= 2 ? 6 : 1), false, true>),
+          dim3(grid), dim3(kBlock), 0, s, a);
+    else if (a.uniform)
       hipLaunchKernelGGL((aeif_pass1_kernel<P, true, (P <= 2 ? 6 : 1), false>),
                          dim3(grid), dim3(kBlock), 0, s, a);
     else
@@ -1800,6 +1910,19 @@ void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream) {
   else
     hipLaunchKernelGGL(delivery_kernel<false>, dim3(grid), dim3(kBlock), lds,
                        (hipStream_t)stream, a);
+}
+
+void LaunchDeliveryTiled(const DeliveryArgs &a, const TileArgs &b,
+                         void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t lds = sizeof(uint32_t) * ((size_t)a.D + 1);
+  hipLaunchKernelGGL(tiled_delivery_kernel, dim3((unsigned)(b.T * b.G)),
+                     dim3(kTiledBlock), lds, s, a, b);
+  long long g = (b.acc_n + kBlock - 1) / kBlock;
+  if (g > 2048) g = 2048;
+  if (b.G <= 1) g = 1;
+  hipLaunchKernelGGL(tiled_sum_kernel, dim3((unsigned)g), dim3(kBlock), 0, s,
+                     b, a.acc, a.clock, a.clock_next);
 }
 
 void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,

@@ -392,6 +392,29 @@ struct BinArgs {
   long long acc_n;          // accumulator entries (ports x nodes)
 };
 
+// Tiled delivery (non-plastic networks): a tile-major copy of the synapse
+// store (records grouped by accumulator tile of kTileLen entries, canonical
+// order inside a tile) and, per (source, delay slot, tile), the first
+// record of that group.  A block owns one tile in LDS: it walks every due
+// (spike, delay) pair's records for its tile, sums them in LDS, and adds
+// the tile to the accumulators (G == 1) or writes it as a partial that a
+// second launch sums (G blocks per tile).  Each record is read once and
+// the accumulators are written once per entry.
+constexpr int kTileShift = 15;
+constexpr int kTileLen = 1 << kTileShift;   // accumulator entries per tile
+constexpr int kTiledBlock = 1024;
+
+struct TileArgs {
+  const unsigned long long *trec;  // [S] tile-major records
+  const uint32_t *off;             // [(N*D + 1) * T]: off[(s*D + d)*T + t]
+                                   // = first record of (s, d) in tile t;
+                                   // the next row ends it
+  float *part;                     // [T*G][kTileLen] partial tiles (G > 1)
+  long long acc_n;                 // accumulator entries (ports x nodes)
+  int T;                           // tiles
+  int G;                           // blocks per tile
+};
+
 struct RevArgs {
   const uint32_t *ring_src;
   long long cap;
@@ -452,6 +475,8 @@ void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream);
 void LaunchClockSet(StepClock *c, const StepClock &v, void *stream);
 // Binned delivery: count, scan, scatter, reduce, sum (five launches).
 // Requires a non-plastic network and b.nb <= kMaxBins.
+void LaunchDeliveryTiled(const DeliveryArgs &a, const TileArgs &b,
+                         void *stream);
 void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,
                           int reduce_grid, void *stream);
 void LaunchRevStdp(const RevArgs &a, int grid, void *stream);

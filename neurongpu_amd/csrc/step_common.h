@@ -92,6 +92,20 @@ __device__ __forceinline__ void emit_spikes(const SpikeOut &o,
   }
 }
 
+// Poisson count of a uniform draw u: the number of the table's
+// nondecreasing thresholds below u (the table ends with 0xFFFFFFFF and is
+// padded with it to kPoissonTableMax entries).  The first 16 thresholds
+// are compared at once, with no load depending on a comparison; a longer
+// table continues in a loop.
+__device__ __forceinline__ int pois_invert(const uint32_t *tab, uint32_t u) {
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) k += (u > tab[i]) ? 1 : 0;
+  if (k == 16)
+    while (u > tab[k]) k++;
+  return k;
+}
+
 // Poisson count of one direct connection (canonical index j, generator g)
 // for the pending step: 0 outside the generator's window
 // (src/poiss_gen.cu:43-74), else one Philox draw inverted over the
@@ -107,9 +121,7 @@ __device__ __forceinline__ int pois_count(const DirectIn &d, uint32_t j,
   if (ti == 0xFFFFFFFFu) return 0;
   const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
   uint32_t u = dev_philox_w0(j, Tlo, Thi, kPoissonTag, d.key0, d.key1);
-  int k = 0;
-  while (u > tab[k]) k++;
-  return k;
+  return pois_invert(tab, u);
 }
 
 // Generator of the compact form's connection j.
@@ -138,8 +150,13 @@ __device__ __forceinline__ void gather_input(const FoldIn &f,
   if (d.compact) {
     const uint32_t j = (uint32_t)node - d.lo;
     if (j >= d.hi - d.lo) return;
-    const int k = pois_count(d, j, compact_gen(d, j), d.c_delay, Tlo, Thi,
-                             sn.pois_t);
+    // one generator for all (its parameters and table are then read with
+    // scalar loads), or one per node
+    const int k = d.gen_mode == 0
+                      ? pois_count(d, j, d.gen0, d.c_delay, Tlo, Thi,
+                                   sn.pois_t)
+                      : pois_count(d, j, compact_gen(d, j), d.c_delay, Tlo,
+                                   Thi, sn.pois_t);
     if (k > 0) {
       float add = d.c_weight * (float)k;
 #pragma unroll

@@ -291,6 +291,8 @@ def test_network_statistics_vs_fp64(ngpu):
     {"delivery_mode": 1},
     {"delivery_mode": 1, "delivery_bin_capacity": 50000},
     {"delivery_mode": 2, "delivery_auto_events": 100000},
+    {"delivery_mode": 3},
+    {"delivery_mode": 3, "delivery_tile_split": 3},
 ])
 def test_c2_iaf_network_bit_exact(ngpu, settings):
     """c2-shaped iaf_psc_exp network at 10 % size (7717 neurons, 5e6
@@ -404,6 +406,9 @@ def test_ensemble_routing_matches_instances_alone(ngpu):
     {"delivery_mode": 1, "step_overlap": 0},
     {"delivery_mode": 1, "step_overlap": 0, "delivery_streams": 1},
     {"delivery_mode": 1, "delivery_grid": 64, "delivery_tile_max": 4},
+    {"delivery_mode": 3},
+    {"delivery_mode": 3, "delivery_tile_split": 1, "step_overlap": 0},
+    {"delivery_mode": 3, "delivery_tile_split": 7, "delivery_tile_max": 2},
 ])
 def test_launch_shapes_do_not_change_results(ngpu, settings):
     """The autotuner's choices (RK5 two-pass budget and grid, delivery grid,
@@ -432,6 +437,7 @@ def _run_spikes_calls(ngpu, spec, calls, t_ms, settings):
     {"graph_steps": 6},
     {"graph_steps": 16, "delivery_mode": 1},
     {"graph_steps": 4, "rk5_two_pass": 0},
+    {"graph_steps": 16, "delivery_mode": 3},
 ])
 def test_graph_steps_match_eager(ngpu, settings):
     """Steps replayed from captured hipGraphs (the step clock advanced on

@@ -541,7 +541,7 @@ def test_launch_settings(ngpu):
                 "delivery_bin_capacity": 1 << 27,
                 "delivery_auto_events": 500000, "graph_steps": 0,
                 "delivery_streams": 2, "step_overlap": 2,
-                "device_connect": 1}
+                "device_connect": 1, "delivery_tile_split": 0}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)
@@ -550,7 +550,12 @@ def test_launch_settings(ngpu):
     assert ngpu.GetKernelStatus("delivery_bin_capacity") == 1000
     ngpu.SetKernelStatus("graph_steps", 16)
     assert ngpu.GetKernelStatus("graph_steps") == 16
-    for k, bad in (("delivery_mode", 3), ("delivery_tile_max", 3),
+    ngpu.SetKernelStatus("delivery_mode", 3)
+    assert ngpu.GetKernelStatus("delivery_mode") == 3
+    ngpu.SetKernelStatus("delivery_tile_split", 8)
+    assert ngpu.GetKernelStatus("delivery_tile_split") == 8
+    for k, bad in (("delivery_mode", 4), ("delivery_tile_max", 3),
+                   ("delivery_tile_split", -1), ("delivery_tile_split", 65),
                    ("delivery_grid", 0), ("rk5_budget", 0),
                    ("delivery_bin_capacity", 0), ("graph_steps", -1),
                    ("graph_steps", 7), ("delivery_streams", 3),

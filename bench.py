@@ -584,6 +584,10 @@ def main():
                     help="further configurations to measure after c1 and "
                          "the README network (comma list of active_1m, c2,"
                          " c3, c4; empty for none)")
+    ap.add_argument("--set", action="append", default=[],
+                    metavar="NAME=VALUE",
+                    help="kernel int parameter set after each network is "
+                         "built (SetKernelStatus), e.g. delivery_mode=3")
     ap.add_argument("--c4-instances", type=int, default=64)
     ap.add_argument("--c2-scale", type=float, default=1.0)
     ap.add_argument("--c3-scale", type=float, default=1.0)
@@ -620,9 +624,16 @@ def main():
         "c4": c4_build,
     }
 
+    settings = {}
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        settings[k] = int(v)
+
     def run(name, after=None):
         log("building", name)
         neuron, n_syn = builders[name]()
+        for k, v in settings.items():
+            ngpu.SetKernelStatus(k, v)
         r = measure(ngpu, neuron, n_syn, args.steps, args.warmup,
                     args.timing_steps, after_warmup=after,
                     settle_steps=0 if name == "c1" else settle)

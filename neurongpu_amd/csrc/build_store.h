@@ -40,6 +40,14 @@ const char *BuildTiledStore(const unsigned long long *rec,
                             unsigned long long *trec, uint32_t *off,
                             void *stream);
 
+// For the fixed-point tile sums of that store: bound >= the most records
+// onto one accumulator entry, max_w = the largest |weight| (NaN if a
+// weight is NaN).  Returns null on success or a static error message.
+const char *TiledSumBound(const unsigned long long *trec, const uint32_t *off,
+                          uint32_t N, uint32_t D, uint32_t T, int tile_shift,
+                          long long n_total, unsigned long long *bound,
+                          float *max_w, void *stream);
+
 // Synapse counts of the canonical store (device pointers): for nodes lo ..
 // hi-1, in_deg[t - lo] += synapses with target t, out_deg[s - lo] +=
 // synapses with source s, delay_cnt[d] += synapses with source in the range

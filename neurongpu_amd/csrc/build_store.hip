@@ -12,6 +12,10 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "build_store.h"
 
 namespace ngpu {
@@ -117,6 +121,62 @@ __global__ void tile_keys(const unsigned long long *rec, unsigned long long S,
   const long long a = (long long)(tp >> kPortShift) * n_total + (tp & kTargetMask);
   keys[i] = (uint16_t)(a >> tile_shift);
   vals[i] = (uint32_t)i;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t m) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t y = (uint32_t)__shfl_xor((int)m, o);
+    m = y > m ? y : m;
+  }
+  return m;
+}
+
+// Bound of the fixed-point tile sums: block (t, k) counts, in LDS, the
+// records onto each entry of tile t in chunk k of gb of the tile's records
+// and adds its largest count to bound[t] (at least the largest count of
+// any entry of the tile, summed over the chunks); wmax gets the largest
+// |weight| as float bits (NaN above all).
+constexpr int kBoundBlock = 1024;
+__global__ __launch_bounds__(kBoundBlock) void tile_bound(
+    const unsigned long long *trec, const uint32_t *off, unsigned long long nd,
+    uint32_t T, int tile_shift, long long n_total, uint32_t gb,
+    uint32_t *bound, uint32_t *wmax) {
+  extern __shared__ uint32_t s_cnt[];
+  __shared__ uint32_t s_red[2];
+  const uint32_t t = blockIdx.x / gb, k = blockIdx.x - t * gb;
+  const int TL = 1 << tile_shift;
+  for (int i = threadIdx.x; i < TL; i += kBoundBlock) s_cnt[i] = 0u;
+  if (threadIdx.x == 0) s_red[0] = s_red[1] = 0u;
+  __syncthreads();
+  const uint32_t r0 = off[t], n = off[nd * T + t] - r0;
+  const uint32_t c0 = r0 + (uint32_t)((unsigned long long)n * k / gb);
+  const uint32_t c1 = r0 + (uint32_t)((unsigned long long)n * (k + 1) / gb);
+  const long long a0 = (long long)t << tile_shift;
+  uint32_t wb = 0;
+  for (uint32_t j = c0 + threadIdx.x; j < c1; j += kBoundBlock) {
+    const unsigned long long r = trec[j];
+    const uint32_t tp = (uint32_t)r;
+    const long long a =
+        (long long)(tp >> kPortShift) * n_total + (tp & kTargetMask);
+    atomicAdd(&s_cnt[a - a0], 1u);
+    const uint32_t x = (uint32_t)(r >> 32) & 0x7FFFFFFFu;
+    wb = x > wb ? x : wb;
+  }
+  __syncthreads();
+  uint32_t m = 0;
+  for (int i = threadIdx.x; i < TL; i += kBoundBlock)
+    m = s_cnt[i] > m ? s_cnt[i] : m;
+  m = wave_max_u32(m);
+  wb = wave_max_u32(wb);
+  if ((threadIdx.x & 63u) == 0) {
+    atomicMax(&s_red[0], m);
+    atomicMax(&s_red[1], wb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(bound + t, s_red[0]);
+    atomicMax(wmax, s_red[1]);
+  }
 }
 
 __global__ void gather_u64(const uint32_t *idx, const unsigned long long *src,
@@ -342,6 +402,43 @@ const char *BuildTiledStore(const unsigned long long *rec,
                      start.p, nd, T, off);
   if (hipStreamSynchronize(stream) != hipSuccess)
     return "tiled store build failed";
+  return nullptr;
+}
+
+const char *TiledSumBound(const unsigned long long *trec, const uint32_t *off,
+                          uint32_t N, uint32_t D, uint32_t T, int tile_shift,
+                          long long n_total, unsigned long long *bound,
+                          float *max_w, void *stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  *bound = 0;
+  *max_w = 0.0f;
+  const unsigned long long nd = (unsigned long long)N * D;
+  if (T == 0) return nullptr;
+  DTmp<uint32_t> tb(T + 1);
+  if (!tb.p) return "out of device memory";
+  if (hipMemsetAsync(tb.p, 0, (T + 1) * sizeof(uint32_t), stream) !=
+      hipSuccess)
+    return "hipMemsetAsync failed";
+  const size_t lds = sizeof(uint32_t) << tile_shift;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(tile_bound),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(uint32_t) << 15)) != hipSuccess)
+      return "hipFuncSetAttribute failed";
+    attr = true;
+  }
+  const uint32_t gb = 32;
+  hipLaunchKernelGGL(tile_bound, dim3(T * gb), dim3(kBoundBlock), lds, stream,
+                     trec, off, nd, T, tile_shift, n_total, gb, tb.p,
+                     tb.p + T);
+  std::vector<uint32_t> h(T + 1);
+  if (hipMemcpyAsync(h.data(), tb.p, (T + 1) * sizeof(uint32_t),
+                     hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return "tiled bound failed";
+  for (uint32_t t = 0; t < T; t++) *bound = std::max<unsigned long long>(*bound, h[t]);
+  std::memcpy(max_w, &h[T], sizeof(float));
   return nullptr;
 }
 

@@ -276,9 +276,10 @@ class Engine {
   int delivery_grid = 2048;   // blocks of the delivery kernel
   int delivery_tile_max = 64; // largest pairs per wave tile
   // 0: float atomics per event; 1: binned (events grouped by accumulator
-  // range, summed in LDS); non-plastic networks only
-  // 2: auto (binned once the recent steps average delivery_auto_events
-  // synaptic events or more, read from the event counter with a lag)
+  // range, summed in LDS); 3: tiled (PrepareTiled); non-plastic networks
+  // only.  2: auto: once the recent steps average delivery_auto_events
+  // synaptic events or more (read from the event counter with a lag),
+  // tiled where TiledPays(), else binned
   int delivery_mode = 2;
   int delivery_bin_capacity = 1 << 27;  // events in the binned-mode buffer
   int delivery_auto_events = 500000;
@@ -350,10 +351,24 @@ class Engine {
   DBuf<unsigned long long> d_trec;
   DBuf<uint32_t> d_toff;
   DBuf<float> d_tpart;
+  DBuf<long long> d_tpart_q;
+  DBuf<uint32_t> d_ttg, d_tblk;   // blocks of each tile, (tile, share) of each block
+  std::vector<uint32_t> tiled_tg;        // the uploaded tg
   bool tiled_built = false, tiled_failed = false;
-  int tiled_T = 0;
+  int tiled_T = 0, tiled_shift = 0;
+  bool tiled_bound_ok = false;      // the two below are for this copy
+  unsigned long long tiled_max_deg = 0;   // >= most records onto one entry
+  float tiled_max_w = 0.0f;         // largest |weight|
+  int tiled_fixed = 0;              // fixed-point tile sums (where they fit)
   int delivery_tile_split = 0;
+  int delivery_tile_shift = 14;   // log2 of the accumulator entries per tile
   bool PrepareTiled(TileArgs &t);
+  // the tiled delivery visits every (spike, delay) pair once per tile: it
+  // pays where the store's (source, delay, tile) segments average at least
+  // kTiledMinSegment records (c2-shaped networks: ~15; the 1e6-neuron
+  // README network: ~0.4)
+  static constexpr double kTiledMinSegment = 4.0;
+  bool TiledPays() const;
   int max_port = 1;
   int R = 0;
   long long ring_cap = 0;

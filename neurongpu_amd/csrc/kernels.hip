@@ -1235,7 +1235,7 @@ __device__ __forceinline__ unsigned long long deliver_events(
 
 // Dynamic LDS of the delivery kernels: s_pre[D + 1], then (binned
 // phases) one counter per bin.
-extern __shared__ uint32_t s_dyn[];
+extern __shared__ __align__(16) uint32_t s_dyn[];
 
 inline size_t delivery_lds_bytes(int D, int nb) {
   return sizeof(uint32_t) * ((size_t)D + 1 + (size_t)nb);
@@ -1551,42 +1551,81 @@ struct TileLoc {
   }
 };
 
+// FIXED: 64-bit fixed-point sums with LDS integer atomics (no return
+// value, no retry loop); else float sums by compare-and-swap.
+template <bool FIXED>
 __global__ __launch_bounds__(kTiledBlock) void tiled_delivery_kernel(
     DeliveryArgs A0, TileArgs B) {
   const DeliveryArgs A = clocked(A0);
-  __shared__ float s_acc[kTileLen];
   __shared__ uint32_t s_part[kTiledBlock];
-  uint32_t *s_pre = s_dyn;
-  const uint32_t t = blockIdx.x / (uint32_t)B.G;
-  const uint32_t k = blockIdx.x - t * (uint32_t)B.G;
-  for (int i = threadIdx.x; i < kTileLen; i += kTiledBlock) s_acc[i] = 0.0f;
+  __shared__ int s_big;   // a float atomic went to the accumulator
+  // dynamic LDS: the tile (2^tile_shift floats or 64-bit sums), then the
+  // pair prefix
+  const int TL = 1 << B.tile_shift;
+  float *s_acc = reinterpret_cast<float *>(s_dyn);
+  unsigned long long *s_q = reinterpret_cast<unsigned long long *>(s_dyn);
+  uint32_t *s_pre = s_dyn + (FIXED ? 2 * TL : TL);
+  const uint32_t tk = B.blk[blockIdx.x];
+  const uint32_t t = tk & 0xFFFFu, k = tk >> 16;
+  const uint32_t g0 = B.tg[t], G = B.tg[t + 1] - g0;
+  for (int i = threadIdx.x; i < TL; i += kTiledBlock) {
+    if (FIXED)
+      s_q[i] = 0ull;
+    else
+      s_acc[i] = 0.0f;
+  }
+  if (threadIdx.x == 0) s_big = 0;
   // (the prologue's barriers order the clearing before the first add)
   const uint32_t total =
       delivery_prologue<kTiledBlock>(A, blockIdx.x, s_pre, s_part);
-  const long long a0 = (long long)t << kTileShift;
-  const unsigned long long n_events = deliver_events<2, kTiledBlock>(
-      A, k, (uint32_t)B.G, s_pre, total,
-      [&](bool ok, unsigned long long, unsigned long long r, float hj,
-          uint32_t) {
-        if (!ok) return;
-        const uint32_t tp = (uint32_t)r;
-        const float w = __uint_as_float((uint32_t)(r >> 32));
-        const long long idx = (long long)(tp >> kPortShift) * A.n_total +
-                              (tp & kTargetMask);
-        lds_add_f32(&s_acc[idx - a0], hj * w);
-      },
-      B.trec, TileLoc{B.off, (uint32_t)B.T, t});
+  const long long a0 = (long long)t << B.tile_shift;
+  auto add = [&](bool ok, unsigned long long, unsigned long long r, float hj,
+                 uint32_t) {
+    if (!ok) return;
+    const uint32_t tp = (uint32_t)r;
+    const long long idx = (long long)(tp >> kPortShift) * A.n_total +
+                          (tp & kTargetMask);
+    const float v = hj * __uint_as_float((uint32_t)(r >> 32));
+    if (FIXED) {
+      if (fabsf(hj) <= B.h_max) {
+        // exact: v * 2^F is v with a larger exponent, below 2^62
+        const long long q = __float2ll_rn(v * B.q_scale);
+        atomicAdd(&s_q[idx - a0], (unsigned long long)q);
+      } else {
+        unsafeAtomicAdd(A.acc + idx, v);
+        s_big = 1;
+      }
+    } else {
+      lds_add_f32(&s_acc[idx - a0], v);
+    }
+  };
+  const unsigned long long n_events =
+      deliver_events<4, kTiledBlock, decltype(add) &, TileLoc>(
+          A, k, G, s_pre, total, add, B.trec,
+          TileLoc{B.off, (uint32_t)B.T, t});
   __syncthreads();
   const long long lim_ll = B.acc_n - a0;
-  const int lim = lim_ll < kTileLen ? (int)lim_ll : kTileLen;
-  if (B.G == 1) {
+  const int lim = lim_ll < TL ? (int)lim_ll : TL;
+  if (G == 1) {
+    // after a float atomic into this tile's entries the flush adds with
+    // atomics too (they and plain accesses are not ordered)
+    const bool big = FIXED && s_big;
     for (int i = threadIdx.x; i < lim; i += kTiledBlock) {
-      const float x = s_acc[i];
-      if (x != 0.0f) A.acc[a0 + i] += x;
+      const float x =
+          FIXED ? (float)(long long)s_q[i] * B.q_inv : s_acc[i];
+      if (x == 0.0f) continue;
+      if (big)
+        unsafeAtomicAdd(A.acc + a0 + i, x);
+      else
+        A.acc[a0 + i] += x;
     }
+  } else if (FIXED) {
+    long long *pq = B.part_q + (size_t)(g0 + k) * TL;
+    for (int i = threadIdx.x; i < TL; i += kTiledBlock)
+      pq[i] = (long long)s_q[i];
   } else {
-    float *pp = B.part + (size_t)blockIdx.x * kTileLen;
-    for (int i = threadIdx.x; i < kTileLen; i += kTiledBlock) pp[i] = s_acc[i];
+    float *pp = B.part + (size_t)(g0 + k) * TL;
+    for (int i = threadIdx.x; i < TL; i += kTiledBlock) pp[i] = s_acc[i];
   }
   if (lane_id() == 0 && n_events)
     atomicAdd(A.events + (blockIdx.x & (kCounterSlots - 1)), n_events);
@@ -1597,15 +1636,50 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_delivery_kernel(
 __global__ __launch_bounds__(kBlock) void tiled_sum_kernel(
     TileArgs B, float *acc, const StepClock *clock, StepClock *clock_next) {
   clock_step(clock, clock_next);
-  if (B.G <= 1) return;
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
-       i < B.acc_n; i += (long long)gridDim.x * kBlock) {
-    const long long t = i >> kTileShift;
-    const float *pp = B.part + (size_t)t * B.G * kTileLen +
-                      (size_t)(i & (kTileLen - 1));
-    float sum = 0.0f;
-    for (int k = 0; k < B.G; k++) sum += pp[(size_t)k * kTileLen];
-    if (sum != 0.0f) acc[i] += sum;
+  if (!B.split) return;
+  // four entries a thread (16-byte loads; a tile holds a multiple of 4),
+  // the partials of each entry added in block order
+  const long long TL = 1ll << B.tile_shift;
+  const long long n4 = (B.acc_n + 3) >> 2;
+  for (long long i4 = (long long)blockIdx.x * kBlock + threadIdx.x; i4 < n4;
+       i4 += (long long)gridDim.x * kBlock) {
+    const long long i = i4 << 2;
+    const long long t = i >> B.tile_shift;
+    const uint32_t g0 = B.tg[t], G = B.tg[t + 1] - g0;
+    if (G <= 1) continue;   // that tile's block added its sums
+    const size_t o = (size_t)g0 * TL + (size_t)(i & (TL - 1));
+    float sum[4];
+    if (B.fixed) {
+      long long q[4] = {0, 0, 0, 0};
+#pragma unroll 4
+      for (uint32_t k = 0; k < G; k++) {
+        const longlong2 *p =
+            reinterpret_cast<const longlong2 *>(B.part_q + o + (size_t)k * TL);
+        const longlong2 a = p[0], b = p[1];
+        q[0] += a.x;
+        q[1] += a.y;
+        q[2] += b.x;
+        q[3] += b.y;
+      }
+      for (int j = 0; j < 4; j++) sum[j] = (float)q[j] * B.q_inv;
+    } else {
+      float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 8
+      for (uint32_t k = 0; k < G; k++) {
+        const float4 x =
+            *reinterpret_cast<const float4 *>(B.part + o + (size_t)k * TL);
+        s.x += x.x;
+        s.y += x.y;
+        s.z += x.z;
+        s.w += x.w;
+      }
+      sum[0] = s.x;
+      sum[1] = s.y;
+      sum[2] = s.z;
+      sum[3] = s.w;
+    }
+    for (int j = 0; j < 4; j++)
+      if (i + j < B.acc_n && sum[j] != 0.0f) acc[i + j] += sum[j];
   }
 }
 
@@ -1915,12 +1989,33 @@ void LaunchDelivery(const DeliveryArgs &a, int grid, void *stream) {
 void LaunchDeliveryTiled(const DeliveryArgs &a, const TileArgs &b,
                          void *stream) {
   hipStream_t s = (hipStream_t)stream;
-  const size_t lds = sizeof(uint32_t) * ((size_t)a.D + 1);
-  hipLaunchKernelGGL(tiled_delivery_kernel, dim3((unsigned)(b.T * b.G)),
-                     dim3(kTiledBlock), lds, s, a, b);
-  long long g = (b.acc_n + kBlock - 1) / kBlock;
+  const size_t lds =
+      sizeof(uint32_t) * (((size_t)(b.fixed ? 2 : 1) << b.tile_shift) +
+                          (size_t)a.D + 1);
+  static bool lds_attr = false;   // the tile may exceed the default limit
+  if (!lds_attr) {
+    // the largest tiles: 2^15 floats, or 2^14 64-bit sums (the host uses
+    // fixed-point sums only where the tile fits in LDS)
+    const int most = (int)(sizeof(uint32_t) *
+                           (((size_t)1 << kTileShiftMax) + kMaxDelaySlots + 1));
+    (void)hipFuncSetAttribute(
+        reinterpret_cast<const void *>(tiled_delivery_kernel<false>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, most);
+    (void)hipFuncSetAttribute(
+        reinterpret_cast<const void *>(tiled_delivery_kernel<true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, most);
+    lds_attr = true;
+  }
+  if (b.fixed)
+    hipLaunchKernelGGL(tiled_delivery_kernel<true>, dim3((unsigned)b.n_blocks),
+                       dim3(kTiledBlock), lds, s, a, b);
+  else
+    hipLaunchKernelGGL(tiled_delivery_kernel<false>,
+                       dim3((unsigned)b.n_blocks), dim3(kTiledBlock), lds, s,
+                       a, b);
+  long long g = ((b.acc_n + 3) / 4 + kBlock - 1) / kBlock;
   if (g > 2048) g = 2048;
-  if (b.G <= 1) g = 1;
+  if (!b.split) g = 1;
   hipLaunchKernelGGL(tiled_sum_kernel, dim3((unsigned)g), dim3(kBlock), 0, s,
                      b, a.acc, a.clock, a.clock_next);
 }

@@ -393,15 +393,15 @@ struct BinArgs {
 };
 
 // Tiled delivery (non-plastic networks): a tile-major copy of the synapse
-// store (records grouped by accumulator tile of kTileLen entries, canonical
+// store (records grouped by accumulator tile of 2^tile_shift entries, canonical
 // order inside a tile) and, per (source, delay slot, tile), the first
 // record of that group.  A block owns one tile in LDS: it walks every due
 // (spike, delay) pair's records for its tile, sums them in LDS, and adds
 // the tile to the accumulators (G == 1) or writes it as a partial that a
 // second launch sums (G blocks per tile).  Each record is read once and
-// the accumulators are written once per entry.
-constexpr int kTileShift = 15;
-constexpr int kTileLen = 1 << kTileShift;   // accumulator entries per tile
+// the accumulators are written once per entry.  The tile holds
+// 2^tile_shift entries (at most 2^15, 128 KB of LDS).
+constexpr int kTileShiftMax = 15;
 constexpr int kTiledBlock = 1024;
 
 struct TileArgs {
@@ -409,11 +409,30 @@ struct TileArgs {
   const uint32_t *off;             // [(N*D + 1) * T]: off[(s*D + d)*T + t]
                                    // = first record of (s, d) in tile t;
                                    // the next row ends it
-  float *part;                     // [T*G][kTileLen] partial tiles (G > 1)
+  // tile t is summed by G_t = tg[t+1] - tg[t] blocks; block b takes
+  // share k = blk[b] >> 16 of the pairs for tile t = blk[b] & 0xFFFF (the
+  // host orders the blocks so that the blocks of one share of every tile
+  // run on one XCD and share its L2).  A tile of one block adds its sums
+  // to the accumulators, else its block k writes partial tile tg[t] + k,
+  // part[tg[t] + k][2^tile_shift]
+  const uint32_t *tg;              // [T + 1]
+  const uint32_t *blk;             // [n_blocks]
+  float *part;
+  long long *part_q;               // the same as fixed-point sums
   long long acc_n;                 // accumulator entries (ports x nodes)
   int T;                           // tiles
-  int G;                           // blocks per tile
+  int n_blocks;                    // tg[T]
+  int split;                       // some tile has more than one block
+  int tile_shift;
+  // fixed != 0: the tile sums height*weight in 64-bit fixed point,
+  // round(height*weight * q_scale) with q_scale = 2^F and q_inv = 2^-F
+  // (exact sums, independent of the order of the events; F leaves the
+  // largest possible sum of an entry below 2^62); an event with
+  // |height| > h_max is added with a float atomic instead
+  int fixed;
+  float q_scale, q_inv, h_max;
 };
+constexpr float kTileHeightMax = 1024.0f;
 
 struct RevArgs {
   const uint32_t *ring_src;

@@ -36,6 +36,18 @@ typedef float REAL;
 #define SQRTR(x) sqrtf(x)
 #endif
 
+/* input accumulator type: REAL, or double with ORC_ACC_DOUBLE (the
+ * reference accumulates the step's synaptic input in double,
+ * src/get_spike.cu:70-72 and :134-138) */
+#ifndef ORC_ACC_DOUBLE
+#define ORC_ACC_DOUBLE 0
+#endif
+#if ORC_ACC_DOUBLE
+typedef double ACC;
+#else
+typedef REAL ACC;
+#endif
+
 #define MAX_PORT 20
 #define MAX_ATTEMPTS_PER_STEP 100000
 #define POISSON_TABLE_MAX 256
@@ -558,7 +570,7 @@ struct orc_net {
   long *dir_conn;   /* index into conn */
   /* per node */
   long long *last_spike, *last_rev;
-  REAL *acc;   /* [max_port][n_node] */
+  ACC *acc;    /* [max_port][n_node] */
   int max_port;
   int *den_idx;
   int max_den;
@@ -950,7 +962,7 @@ int orc_net_calibrate(orc_net *net) {
     net->last_rev[i] = LAST_SPIKE_TIME_GUARD;
   }
   int mp = net->max_port > 0 ? net->max_port : 1;
-  net->acc = (REAL *)calloc((size_t)mp * net->n_node, sizeof(REAL));
+  net->acc = (ACC *)calloc((size_t)mp * net->n_node, sizeof(ACC));
   net->den_idx = (int *)calloc(net->n_node, sizeof(int));
   net->last_push = (long *)malloc(net->n_node * sizeof(long));
   for (int i = 0; i < net->n_node; i++) net->last_push[i] = -1;
@@ -1494,7 +1506,7 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
   }
   record_row(net, t);
   /* phase 4: delivery, src/get_spike.cu:48-85 */
-  REAL *acc = net->acc;
+  ACC *acc = net->acc;
   for (int d = 0; d < net->max_d; d++) {
     long s = net->gs - 1 - d;
     if (s < 0) break;
@@ -1511,7 +1523,7 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
         if (cn->d != d) break;
         net->cnt_events++;
         REAL w = (REAL)cn->weight;
-        acc[(size_t)cn->port * nn + cn->tgt] += sp->height * w;
+        acc[(size_t)cn->port * nn + cn->tgt] += (ACC)(sp->height * w);
         if (cn->syn_group > 0) {
           for (int rep = 0; rep < sp->count; rep++) {
             net->spike_time[c] = (uint16_t)(T & 0xffff);
@@ -1544,7 +1556,7 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
         int n = orc_poisson_sample(tab, out[0]);
         net->cnt_pois++;
         if (n > 0)
-          acc[(size_t)cn->port * nn + cn->tgt] += (REAL)(cn->weight * (float)n);
+          acc[(size_t)cn->port * nn + cn->tgt] += (ACC)(cn->weight * (float)n);
       }
     }
   }
@@ -1556,7 +1568,7 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
       int g0i = g->model == ORC_AEIF_COND_BETA ? 3 : 2;
       for (int q = 0; q < g->n_port; q++)
         for (int i = 0; i < g->n; i++) {
-          REAL a = acc[(size_t)q * nn + g->i0 + i];
+          REAL a = (REAL)acc[(size_t)q * nn + g->i0 + i];
           REAL *g1 = V(g, 3 + 2 * q, i);
           *g1 = FMA(a, *P(g, 14 + pp * q + g0i, i), *g1);
         }
@@ -1564,7 +1576,7 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
       /* I1_syn += input * I0 */
       for (int q = 0; q < g->n_port; q++)
         for (int i = 0; i < g->n; i++) {
-          REAL a = acc[(size_t)q * nn + g->i0 + i];
+          REAL a = (REAL)acc[(size_t)q * nn + g->i0 + i];
           REAL *I1 = V(g, 3 + 2 * q, i);
           *I1 = FMA(a, *P(g, 14 + 2 * q + 1, i), *I1);
         }
@@ -1573,19 +1585,19 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
       for (int q = 0; q < g->n_port; q++)
         for (int i = 0; i < g->n; i++) {
           REAL *I = V(g, 2 + q, i);
-          *I = *I + acc[(size_t)q * nn + g->i0 + i];
+          *I = *I + (REAL)acc[(size_t)q * nn + g->i0 + i];
         }
     } else if (g->model == ORC_AEIF_PSC_DELTA) {
       /* every port into V_m (weight 1, src/aeif_psc_delta.cu:112-120) */
       for (int q = 0; q < g->n_port; q++)
         for (int i = 0; i < g->n; i++) {
           REAL *Vm = V(g, 0, i);
-          *Vm = *Vm + acc[(size_t)q * nn + g->i0 + i];
+          *Vm = *Vm + (REAL)acc[(size_t)q * nn + g->i0 + i];
         }
     } else if (g->model == ORC_IZH_COND_BETA) {
       for (int q = 0; q < g->n_port; q++)
         for (int i = 0; i < g->n; i++) {
-          REAL a = acc[(size_t)q * nn + g->i0 + i];
+          REAL a = (REAL)acc[(size_t)q * nn + g->i0 + i];
           REAL *g1 = V(g, 3 + 2 * q, i);
           *g1 = FMA(a, *P(g, 9 + 4 * q + 3, i), *g1);
         }
@@ -1593,30 +1605,30 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
       for (int q = 0; q < 2; q++)
         for (int i = 0; i < g->n; i++) {
           REAL *I = V(g, q, i);
-          *I = *I + acc[(size_t)q * nn + g->i0 + i];
+          *I = *I + (REAL)acc[(size_t)q * nn + g->i0 + i];
         }
     } else if (g->model == ORC_EXT_NEURON) {
       /* port_input += input * port_weight */
       for (int q = 0; q < g->n_port; q++)
         for (int i = 0; i < g->n; i++) {
           REAL *in = V(g, 2 * q, i);
-          *in = FMA(acc[(size_t)q * nn + g->i0 + i], *P(g, 1 + q, i), *in);
+          *in = FMA((REAL)acc[(size_t)q * nn + g->i0 + i], *P(g, 1 + q, i), *in);
         }
     } else if (g->model >= ORC_IAF_PSC_EXP_G && g->model <= ORC_IZH_PSC_EXP_5S) {
       /* one port, weight 1, into I_syn */
       for (int i = 0; i < g->n; i++) {
         REAL *I = V(g, 0, i);
-        *I = *I + acc[g->i0 + i];
+        *I = *I + (REAL)acc[g->i0 + i];
       }
     } else if (g->model == ORC_PARROT_NEURON) {
       /* port 0 weight 1 into input_spike_height; port 1 weight 0 */
       for (int i = 0; i < g->n; i++) {
         REAL *I = V(g, 0, i);
-        *I = *I + acc[g->i0 + i];
+        *I = *I + (REAL)acc[g->i0 + i];
       }
     }
   }
-  memset(acc, 0, (size_t)(net->max_port > 0 ? net->max_port : 1) * nn * sizeof(REAL));
+  memset(acc, 0, (size_t)(net->max_port > 0 ? net->max_port : 1) * nn * sizeof(ACC));
   /* phase 7: post-synaptic STDP, src/rev_spike.cu:41-75 */
   if (net->any_plastic) {
     for (int node = 0; node < nn; node++) {

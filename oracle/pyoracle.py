@@ -98,7 +98,8 @@ STDP_DEFAULT = {"tau_plus": 20.0, "tau_minus": 20.0, "lambda": 1.0e-4,
 
 
 def ensure_built():
-    for name in ("liboracle_f32.so", "liboracle_f64.so"):
+    for name in ("liboracle_f32.so", "liboracle_f64.so",
+                 "liboracle_f32a64.so"):
         if not os.path.exists(os.path.join(HERE, name)):
             subprocess.check_call(["make", "-s", "-C", HERE])
             break
@@ -303,7 +304,8 @@ class Net:
     def __init__(self, dt=0.1, t0=0.0, seed=54321, prec="f32"):
         self.L = lib(prec)
         self.prec = prec
-        self.dtype = np.float32 if prec == "f32" else np.float64
+        # state precision (f32a64: float state, double input accumulators)
+        self.dtype = np.float64 if prec == "f64" else np.float32
         self.h = self.L.orc_net_create(dt, t0, seed)
         self.groups = []
         self.dt = dt

@@ -293,6 +293,7 @@ def test_network_statistics_vs_fp64(ngpu):
     {"delivery_mode": 2, "delivery_auto_events": 100000},
     {"delivery_mode": 3},
     {"delivery_mode": 3, "delivery_tile_split": 3},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1},
 ])
 def test_c2_iaf_network_bit_exact(ngpu, settings):
     """c2-shaped iaf_psc_exp network at 10 % size (7717 neurons, 5e6
@@ -409,6 +410,10 @@ def test_ensemble_routing_matches_instances_alone(ngpu):
     {"delivery_mode": 3},
     {"delivery_mode": 3, "delivery_tile_split": 1, "step_overlap": 0},
     {"delivery_mode": 3, "delivery_tile_split": 7, "delivery_tile_max": 2},
+    {"delivery_mode": 3, "delivery_tile_shift": 10},
+    {"delivery_mode": 3, "delivery_tile_shift": 11, "delivery_tile_split": 1},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_split": 3},
 ])
 def test_launch_shapes_do_not_change_results(ngpu, settings):
     """The autotuner's choices (RK5 two-pass budget and grid, delivery grid,
@@ -760,6 +765,86 @@ def test_point_model_network_bit_exact(ngpu, model):
     log("spikes engine", len(eng), "oracle", len(orc))
     assert len(orc) > 200
     assert eng == orc
+
+
+@pytest.mark.parametrize("settings", [
+    {"delivery_mode": 3},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_split": 3},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_shift": 10},
+])
+def test_tiled_delivery_spike_heights(ngpu, settings):
+    """Tiled delivery with spike heights other than 1: the 500-neuron
+    iaf_psc_exp_g network plus four spike generators onto every neuron,
+    with heights 1, 3, 0.5 and 2000 (above the fixed-point tile's height
+    bound, so added with a float atomic beside the tile's exact sums);
+    integer and dyadic weights keep every sum exact, so the spike trains
+    equal the FP32 oracle's spike for spike over 200 ms."""
+    log("case tiled_spike_heights", settings)
+    n = 500
+    spec = _point_net_spec("iaf_psc_exp_g", n=n)
+    spec["groups"].append({"model": "spike_generator", "n": 4})
+    rng = np.random.default_rng(3)
+    heights = [1.0, 3.0, 0.5, 2000.0]
+    spec["spike_gen"] = []
+    for g, h in enumerate(heights):
+        times = np.sort(rng.choice(np.arange(5, 1900), 40, replace=False))
+        times = [float(t) * 0.1 for t in times]
+        spec["spike_gen"].append({"node": n + g, "times": times,
+                                  "heights": [h] * len(times)})
+        w = rng.integers(-8, 9, n).astype(np.float32) * np.float32(
+            0.25 if h > 100 else 4.0)
+        spec["conns"].append({"src": np.full(n, n + g), "tgt": np.arange(n),
+                              "weight": w,
+                              "delay": rng.integers(1, 31, n) * 0.1,
+                              "port": 0})
+    groups, _ = nb.build_engine(ngpu, spec)
+    for k, v in settings.items():
+        ngpu.SetKernelStatus(k, v)
+    ngpu.ActivateRecSpikeTimes(groups[0], 2000)
+    ngpu.Simulate(200.0)
+    eng = sorted(nb.engine_spikes(ngpu, groups[0]))
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(2000)
+    orc = sorted(nb.oracle_spikes(net, 0.1, 0, n))
+    log("spikes engine", len(eng), "oracle", len(orc))
+    assert len(orc) > 200
+    assert eng == orc
+
+
+def test_tiled_fixed_point_sums_are_deterministic(ngpu):
+    """Tiled delivery with weights that are not dyadic (float sums depend on
+    the order of the adds): the fixed-point tile gives the same V_m, bit
+    for bit, on two runs and with 1 or 5 blocks per tile, and stays within
+    float rounding of the per-event atomic delivery."""
+    log("case tiled_fixed_deterministic")
+    spec = nb.ensemble_spec(_ensemble_instances())
+    rng = np.random.default_rng(5)
+    for c in spec["conns"]:
+        w = np.asarray(c["weight"], np.float32)
+        if w.ndim == 1:
+            c["weight"] = w * rng.uniform(0.9, 1.1, w.size).astype(np.float32)
+
+    def run(settings):
+        groups, _ = nb.build_engine(ngpu, spec)
+        for k, v in settings.items():
+            ngpu.SetKernelStatus(k, v)
+        ngpu.Simulate(10.0)
+        n = groups[0].n
+        return (np.asarray(ngpu.GetNeuronVar(0, n, "V_m"), np.float32),
+                np.asarray(ngpu.GetNeuronVar(0, n, "g1"), np.float32))
+
+    fx = {"delivery_mode": 3, "delivery_tile_fixed": 1}
+    a = run(dict(fx, delivery_tile_split=1))
+    b = run(dict(fx, delivery_tile_split=5))
+    c = run(dict(fx, delivery_tile_split=5))
+    z = run({"delivery_mode": 0})
+    for x, y in ((a, b), (b, c)):
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+    log("max |V_m - atomic|", float(np.max(np.abs(a[0] - z[0]))),
+        "max |g1 - atomic|", float(np.max(np.abs(a[1] - z[1]))))
+    np.testing.assert_allclose(a[0], z[0], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(a[1], z[1], rtol=1e-4, atol=1e-6)
 
 
 def test_ext_neuron_bit_exact(ngpu):

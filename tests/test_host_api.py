@@ -541,7 +541,8 @@ def test_launch_settings(ngpu):
                 "delivery_bin_capacity": 1 << 27,
                 "delivery_auto_events": 500000, "graph_steps": 0,
                 "delivery_streams": 2, "step_overlap": 2,
-                "device_connect": 1, "delivery_tile_split": 0}
+                "device_connect": 1, "delivery_tile_split": 0,
+                "delivery_tile_shift": 14, "delivery_tile_fixed": 0}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)
@@ -556,10 +557,11 @@ def test_launch_settings(ngpu):
     assert ngpu.GetKernelStatus("delivery_tile_split") == 8
     for k, bad in (("delivery_mode", 4), ("delivery_tile_max", 3),
                    ("delivery_tile_split", -1), ("delivery_tile_split", 65),
+                   ("delivery_tile_shift", 9), ("delivery_tile_shift", 16),
                    ("delivery_grid", 0), ("rk5_budget", 0),
                    ("delivery_bin_capacity", 0), ("graph_steps", -1),
                    ("graph_steps", 7), ("delivery_streams", 3),
                    ("step_overlap", -1), ("step_overlap", 1025),
-                   ("device_connect", 2)):
+                   ("device_connect", 2), ("delivery_tile_fixed", 2)):
         with pytest.raises(ValueError):
             ngpu.SetKernelStatus(k, bad)

@@ -664,7 +664,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded network, Philox Poisson drive)",
+        "data": "synthetic (seeded network, Threefry Poisson drive)",
         "config": {"workload": "c1: 1M aeif_cond_beta, 1e9 synapses "
                                "(800 E w=0.05, 200 I w=0.25), delays 1-20 "
                                "steps, 8 kHz Poisson w=0.1, RK5 tol 1e-3, "

@@ -90,16 +90,16 @@ __device__ __forceinline__ void aeif_rk5_begin(const AeifLocal<P> &p,
                                                const float (&y)[2 + 2 * P],
                                                float h,
                                                float (&k1)[2 + 2 * P],
-                                               float (&inv_scal)[2 + 2 * P]) {
-  rk5_begin(AeifModel<P>{p}, refr, y, h, k1, inv_scal);
+                                               float (&scal)[2 + 2 * P]) {
+  rk5_begin(AeifModel<P>{p}, refr, y, h, k1, scal);
 }
 
 template <int P>
 __device__ __forceinline__ float aeif_rk5_attempt(
     const AeifLocal<P> &p, float refr, const float (&y)[2 + 2 * P],
-    const float (&k1)[2 + 2 * P], const float (&inv_scal)[2 + 2 * P],
+    const float (&k1)[2 + 2 * P], const float (&scal)[2 + 2 * P],
     float hh, float (&yn)[2 + 2 * P]) {
-  return rk5_attempt(AeifModel<P>{p}, refr, y, k1, inv_scal, hh, yn);
+  return rk5_attempt(AeifModel<P>{p}, refr, y, k1, scal, hh, yn);
 }
 
 template <int P>
@@ -403,7 +403,7 @@ template <int P>
 __device__ __forceinline__ bool aeif_attempt_and_update(
     const AeifArgs &A, const double x1, const AeifLocal<P> &p,
     float (&y)[2 + 2 * P], const float (&k1)[2 + 2 * P],
-    const float (&inv_scal)[2 + 2 * P], double &x, float &h, float &refr,
+    const float (&scal)[2 + 2 * P], double &x, float &h, float &refr,
     int &nsp, int &n_att, bool &need_k1, int &n_acc, int &n_rej,
     float &err_out, float &hh_out) {
   constexpr int N = 2 + 2 * P;
@@ -413,7 +413,7 @@ __device__ __forceinline__ bool aeif_attempt_and_update(
   if (hh > h_max) hh = h_max;
   if (hh < A.h_min) hh = A.h_min;
   h = hh;
-  float err = aeif_rk5_attempt<P>(p, refr, y, k1, inv_scal, hh, yn);
+  float err = aeif_rk5_attempt<P>(p, refr, y, k1, scal, hh, yn);
   err = err / A.eps;
   err_out = err;
   hh_out = hh;
@@ -548,7 +548,7 @@ void aeif_pass1_kernel(AeifArgs A) {
   } else if (active) {
     AeifLocal<P> p;
     aeif_load_params<P, UNI>(A, i, p);
-    float y[N], k1[N], inv_scal[N];
+    float y[N], k1[N], scal[N];
     const uint32_t u = (uint32_t)i;
     y[0] = at(A.V, u);
     y[1] = at(A.w, u);
@@ -572,10 +572,10 @@ void aeif_pass1_kernel(AeifArgs A) {
       if (n_att >= (B1 ? 1 : A.budget)) break;
       if (need_k1) {
         h_begin = h;
-        aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
+        aeif_rk5_begin<P>(p, refr, y, h, k1, scal);
         need_k1 = false;
       }
-      if (aeif_attempt_and_update<P>(A, x1, p, y, k1, inv_scal, x, h, refr,
+      if (aeif_attempt_and_update<P>(A, x1, p, y, k1, scal, x, h, refr,
                                      nsp, n_att, need_k1, n_acc, n_rej,
                                      err_last, hh_used)) {
         finished = true;
@@ -683,7 +683,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
   const StragRec *list_prev = list + (size_t)kStragLists * A.strag_cap;
   const float *carry = A.carry + (size_t)(gs & 1) * P * A.n;
   AeifLocal<P> p;
-  float y[N], k1[N], inv_scal[N];
+  float y[N], k1[N], scal[N];
   double x = 0.0;
   float h = 0.0f, refr = 0.0f;
   int nsp = 0, n_att = 0, n_acc = 0, n_rej = 0;
@@ -733,7 +733,7 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
       } else if (r.flags & 1u) {
         need_k1 = true;  // saved at a step boundary: a new RK5 step starts
       } else {
-        aeif_rk5_begin<P>(p, refr, y, r.h_begin, k1, inv_scal);
+        aeif_rk5_begin<P>(p, refr, y, r.h_begin, k1, scal);
         need_k1 = false;
       }
       have = true;
@@ -763,11 +763,11 @@ __device__ __forceinline__ void aeif_pass2_body(const AeifArgs &A,
     }
     if (have) {
       if (need_k1) {
-        aeif_rk5_begin<P>(p, refr, y, h, k1, inv_scal);
+        aeif_rk5_begin<P>(p, refr, y, h, k1, scal);
         need_k1 = false;
       }
       if (A.prop_log) prop_features<P>(p, y, h, err_last, 1.0f / A.dt, feat);
-      finished = aeif_attempt_and_update<P>(A, x1, p, y, k1, inv_scal, x, h,
+      finished = aeif_attempt_and_update<P>(A, x1, p, y, k1, scal, x, h,
                                             refr, nsp, n_att, need_k1, n_acc,
                                             n_rej, err, hh);
       err_last = err;
@@ -977,8 +977,9 @@ __global__ __launch_bounds__(kBlock) void spikegen_kernel(SpikeGenArgs A) {
 }
 
 // ---------------------------------------------------------------------
-// CreatePoissonGenerator nodes (src/poisson.cu:27-54): Philox word at
-// counter (node, T, kLegacyPoissonTag), inverted through the CDF table.
+// CreatePoissonGenerator nodes (src/poisson.cu:27-54): Threefry word at
+// counter (node, T mod 2^32), key (seed_lo ^ kLegacyPoissonTag, seed_hi +
+// T / 2^32), inverted through the CDF table.
 // The reference pushes these spikes before it advances the step's time
 // (src/neurongpu.cu:461-470), so they carry the previous step's time index
 // and neural time; they enter the same ring slot as this step's spikes.
@@ -998,9 +999,9 @@ __global__ __launch_bounds__(kBlock) void legacy_pois_kernel(LegacyPoisArgs A) {
   int nsp = 0;
   float height = 0.0f;
   if (active) {
-    uint32_t u = dev_philox_w0((uint32_t)node, (uint32_t)T,
-                               (uint32_t)((uint64_t)T >> 32),
-                               kLegacyPoissonTag, A.key0, A.key1);
+    uint32_t u = dev_threefry2x32_w0(
+        (uint32_t)node, (uint32_t)T, A.key0 ^ kLegacyPoissonTag,
+        A.key1 + (uint32_t)((uint64_t)T >> 32));
     int k = 0;
     while (u > A.tab[k]) k++;   // the table ends with 0xFFFFFFFF
     if ((float)k > A.max_count) k = 0;

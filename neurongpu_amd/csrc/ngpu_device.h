@@ -115,6 +115,32 @@ struct CK {
   static constexpr float scal_min = (float)0.1;
 };
 
+// Threefry2x32-20 (Salmon, Moraes, Dror, Shaw, SC'11: the Threefish
+// rounds of Skein, rotations 13 15 26 6 17 29 16 24, key injection every
+// four rounds with the parity word 0x1BD11BDA); returns word 0.  Adds,
+// rotations and xors only: the Poisson draw of every neuron-step uses it,
+// where Philox's 32-bit multiplies (quarter rate) would cost more.
+__device__ __forceinline__ uint32_t dev_threefry2x32_w0(uint32_t c0,
+                                                        uint32_t c1,
+                                                        uint32_t k0,
+                                                        uint32_t k1) {
+  const uint32_t ks[3] = {k0, k1, 0x1BD11BDAu ^ k0 ^ k1};
+  constexpr int rot[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+  uint32_t x0 = c0 + ks[0], x1 = c1 + ks[1];
+#pragma unroll
+  for (int r = 0; r < 20; r++) {
+    x0 += x1;
+    x1 = (x1 << rot[r & 7]) | (x1 >> (32 - rot[r & 7]));
+    x1 ^= x0;
+    if ((r & 3) == 3) {
+      const uint32_t s = (uint32_t)(r + 1) >> 2;
+      x0 += ks[s % 3];
+      x1 += ks[(s + 1) % 3] + s;
+    }
+  }
+  return x0;
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11); returns word 0.
 __device__ __forceinline__ uint32_t dev_philox_w0(uint32_t c0, uint32_t c1,
                                                   uint32_t c2, uint32_t c3,

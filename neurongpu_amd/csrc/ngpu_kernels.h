@@ -322,7 +322,7 @@ struct LegacyPoisArgs {
   int n, i0;
   const uint32_t *tab;          // CDF thresholds of Poisson(lambda)
   float max_count;              // 5 lambda
-  uint32_t key0, key1;          // Philox key (the random seed)
+  uint32_t key0, key1;          // Threefry key (the random seed)
   long long it;                 // step index within this Simulate call
   double t0, dt;                // neural time at the call's start, time step
   SpikeOut out;

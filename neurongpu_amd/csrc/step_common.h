@@ -108,7 +108,8 @@ __device__ __forceinline__ int pois_invert(const uint32_t *tab, uint32_t u) {
 
 // Poisson count of one direct connection (canonical index j, generator g)
 // for the pending step: 0 outside the generator's window
-// (src/poiss_gen.cu:43-74), else one Philox draw inverted over the
+// (src/poiss_gen.cu:43-74), else one Threefry draw (counter (j, T mod
+// 2^32), key (seed_lo ^ tag, seed_hi + T / 2^32)) inverted over the
 // generator's CDF table.
 __device__ __forceinline__ int pois_count(const DirectIn &d, uint32_t j,
                                           uint32_t g, float delay,
@@ -120,7 +121,8 @@ __device__ __forceinline__ int pois_count(const DirectIn &d, uint32_t j,
   uint32_t ti = d.gen_table[g];
   if (ti == 0xFFFFFFFFu) return 0;
   const uint32_t *tab = d.tables + (size_t)ti * kPoissonTableMax;
-  uint32_t u = dev_philox_w0(j, Tlo, Thi, kPoissonTag, d.key0, d.key1);
+  uint32_t u = dev_threefry2x32_w0(j, Tlo, d.key0 ^ kPoissonTag,
+                                   d.key1 + Thi);
   return pois_invert(tab, u);
 }
 
@@ -210,29 +212,33 @@ __device__ __forceinline__ float gather_port(const FoldIn &f, long long pois_T,
 
 // Start of an adaptive Cash-Karp step (src/rk5.h:57-69) of model M (M::N
 // state variables, m.rhs(refr, y, dy) the derivatives): first stage and
-// the error scale, which stay fixed over the step's retries.
+// the error scale |y| + |k1 h| + scal_min, which stay fixed over the
+// step's retries.
 template <class M>
 __device__ __forceinline__ void rk5_begin(const M &m, float refr,
                                           const float (&y)[M::N], float h,
                                           float (&k1)[M::N],
-                                          float (&inv_scal)[M::N]) {
+                                          float (&scal)[M::N]) {
   constexpr int N = M::N;
   m.rhs(refr, y, k1);
 #pragma unroll
   for (int i = 0; i < N; i++) {
     float s = fabsf(y[i]) + fabsf(k1[i] * h);
-    s = s + CK::scal_min;
-    inv_scal[i] = 1.0f / s;
+    scal[i] = s + CK::scal_min;
   }
 }
 
 // One attempt of the step with size hh (src/rk5.h:70-135): stages 2-6,
-// the fifth-order solution in yn and the scaled error norm (before the
-// division by the tolerance).
+// the fifth-order solution in yn and the scaled error norm
+// max_i |hh e_i| / scal_i (before the division by the tolerance).  The
+// largest quotient is found by cross-multiplication (|hh e_i| scal_m
+// against |hh e_m| scal_i, a NaN taking the place, as a NaN quotient
+// does in the reference's running maximum), so that only the winner is
+// divided.
 template <class M>
 __device__ __forceinline__ float rk5_attempt(
     const M &m, float refr, const float (&y)[M::N], const float (&k1)[M::N],
-    const float (&inv_scal)[M::N], float hh, float (&yn)[M::N]) {
+    const float (&scal)[M::N], float hh, float (&yn)[M::N]) {
   constexpr int N = M::N;
   float k2[N], k3[N], k4[N], k5[N], k6[N];
 #pragma unroll
@@ -283,7 +289,7 @@ __device__ __forceinline__ float rk5_attempt(
     t = __builtin_fmaf(CK::a76, k6[i], t);
     yn[i] = __builtin_fmaf(hh, t, y[i]);
   }
-  float err = 0.0f;
+  float num = 0.0f, den = 1.0f;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     float t = CK::e1 * k1[i];
@@ -291,10 +297,12 @@ __device__ __forceinline__ float rk5_attempt(
     t = __builtin_fmaf(CK::e4, k4[i], t);
     t = __builtin_fmaf(CK::e5, k5[i], t);
     t = __builtin_fmaf(CK::e6, k6[i], t);
-    float v = fabsf((hh * t) * inv_scal[i]);
-    err = (err > v) ? err : v;
+    const float v = fabsf(hh * t);
+    const bool take = !(v * den <= num * scal[i]);
+    num = take ? v : num;
+    den = take ? scal[i] : den;
   }
-  return err;
+  return num / den;
 }
 
 // An attempt accepted with a scaled error above 1, which the acceptance
@@ -313,15 +321,15 @@ __device__ __forceinline__ bool rk5_step(const M &m, float refr, double &x,
                                          int &n_acc, int &n_rej,
                                          unsigned long long *forced) {
   constexpr int N = M::N;
-  float k1[N], yn[N], inv_scal[N];
+  float k1[N], yn[N], scal[N];
   float err = 0.0f;
-  rk5_begin(m, refr, y, h, k1, inv_scal);
+  rk5_begin(m, refr, y, h, k1, scal);
   for (;;) {
     float hh = h;
     if (hh > h_max) hh = h_max;
     if (hh < h_min) hh = h_min;
     h = hh;
-    err = rk5_attempt(m, refr, y, k1, inv_scal, hh, yn);
+    err = rk5_attempt(m, refr, y, k1, scal, hh, yn);
     err = err / eps;
     n_att++;
     if (err <= 1.0f || (double)hh <= h_min_acc) break;

@@ -76,6 +76,44 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+/* ------------------------------------------------------------------ */
+/* Threefry2x32-20 (Salmon, Moraes, Dror, Shaw, SC'11): Threefish rounds */
+/* (add, rotate left, xor) with rotations 13 15 26 6 17 29 16 24 and a   */
+/* key injection after every fourth round from the schedule             */
+/* (k0, k1, 0x1BD11BDA ^ k0 ^ k1), plus the injection count on word 1.  */
+/* ------------------------------------------------------------------ */
+void orc_threefry2x32_20(const uint32_t ctr[2], const uint32_t key[2],
+                         uint32_t out[2]) {
+  static const int rot[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+  const uint32_t ks[3] = {key[0], key[1], 0x1BD11BDAu ^ key[0] ^ key[1]};
+  uint32_t x0 = ctr[0] + ks[0], x1 = ctr[1] + ks[1];
+  for (int r = 0; r < 20; r++) {
+    x0 += x1;
+    x1 = (x1 << rot[r % 8]) | (x1 >> (32 - rot[r % 8]));
+    x1 ^= x0;
+    if (r % 4 == 3) {
+      uint32_t s = (uint32_t)(r + 1) / 4;
+      x0 += ks[s % 3];
+      x1 += ks[(s + 1) % 3] + s;
+    }
+  }
+  out[0] = x0;
+  out[1] = x1;
+}
+
+/* Poisson draw word of the direct connections and CreatePoissonGenerator
+ * nodes: Threefry2x32-20 at counter (index, T mod 2^32), key
+ * (seed_lo ^ tag, seed_hi + T / 2^32), word 0. */
+static uint32_t orc_pois_word(uint64_t seed, uint32_t index, long long T,
+                              uint32_t tag) {
+  uint32_t ctr[2] = {index, (uint32_t)T};
+  uint32_t key[2] = {(uint32_t)seed ^ tag,
+                     (uint32_t)(seed >> 32) + (uint32_t)((uint64_t)T >> 32)};
+  uint32_t out[2];
+  orc_threefry2x32_20(ctr, key, out);
+  return out[0];
+}
+
 /* Poisson CDF table: tab[k] = ceil(2^32 F(k)) - 1 clamped to 2^32-1, where
  * F is the Poisson(lambda) CDF computed in double by the pmf recurrence.
  * A draw is X = min{k : u <= tab[k]} for u uniform on 32 bits
@@ -252,14 +290,13 @@ typedef struct {
 static int orc_rk5_step(int nvar, double *x, REAL *y, REAL *h, REAL h_min,
                         REAL h_max, REAL eps, orc_rhs_fn rhs, const void *ctx,
                         orc_rk5_count *cnt) {
-  REAL k1[64], k2[64], k3[64], k4[64], k5[64], k6[64], yn[64], inv_scal[64];
+  REAL k1[64], k2[64], k3[64], k4[64], k5[64], k6[64], yn[64], scal[64];
   REAL err = 0;
   double h_min_acc = (double)h_min * (1.0 + 1.0e-5);
   rhs(nvar, y, k1, ctx);
   for (int i = 0; i < nvar; i++) {
     REAL s = (REAL)fabs((double)y[i]) + (REAL)fabs((double)(k1[i] * (*h)));
-    s = s + scal_min;
-    inv_scal[i] = (REAL)1 / s;
+    scal[i] = s + scal_min;
   }
   for (;;) {
     REAL hh = *h;
@@ -308,16 +345,22 @@ static int orc_rk5_step(int nvar, double *x, REAL *y, REAL *h, REAL h_min,
       t = FMA(a76, k6[i], t);
       yn[i] = FMA(hh, t, y[i]);
     }
-    err = 0;
+    /* max_i |hh t_i| / scal_i: the largest quotient found by
+     * cross-multiplication (a NaN takes the place), only it divided */
+    REAL num = 0, den = 1;
     for (int i = 0; i < nvar; i++) {
       REAL t = e1 * k1[i];
       t = FMA(e3, k3[i], t);
       t = FMA(e4, k4[i], t);
       t = FMA(e5, k5[i], t);
       t = FMA(e6, k6[i], t);
-      REAL v = (REAL)fabs((double)((hh * t) * inv_scal[i]));
-      err = (err > v) ? err : v;
+      REAL v = (REAL)fabs((double)(hh * t));
+      if (!(v * den <= num * scal[i])) {
+        num = v;
+        den = scal[i];
+      }
     }
+    err = num / den;
     err = err / eps;
     cnt->n_att++;
     if (err <= (REAL)1 || (double)hh <= h_min_acc) break;
@@ -1479,14 +1522,11 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
        * above 5 lambda, pushed as the spike height.  The push precedes the
        * time update of the step (src/neurongpu.cu:461-470), so the spike is
        * stamped with the previous time index. */
-      uint32_t key[2] = {(uint32_t)net->seed, (uint32_t)(net->seed >> 32)};
       for (int i = 0; i < g->n; i++) {
         int node = g->i0 + i;
-        uint32_t ctr[4] = {(uint32_t)node, (uint32_t)T,
-                           (uint32_t)((uint64_t)T >> 32), 0x4C504F49u};
-        uint32_t out[4];
-        orc_philox4x32_10(ctr, key, out);
-        int k = orc_poisson_sample(g->lp_tab, out[0]);
+        uint32_t u = orc_pois_word((uint64_t)net->seed, (uint32_t)node, T,
+                                   0x4C504F49u);
+        int k = orc_poisson_sample(g->lp_tab, u);
         if ((float)k > g->lp_lam * 5.0f) k = 0;
         if (k > 0) push_spike(net, node, (REAL)k, T - 1, t);
       }
@@ -1537,7 +1577,6 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
   }
   /* phase 5: Poisson direct connections, src/poiss_gen.cu:43-74 */
   if (net->n_dir > 0) {
-    uint32_t key[2] = {(uint32_t)net->seed, (uint32_t)(net->seed >> 32)};
     float time_step = (float)(net->dt_d / 1000.0);
     for (long j = 0; j < net->n_dir; j++) {
       orc_conn *cn = &net->conn[net->dir_conn[j]];
@@ -1549,11 +1588,9 @@ static void orc_step(orc_net *net, long it, double t0_run, long long T_base,
         (void)time_step;
         if (!net->pg_ok[cn->src]) continue;
         const uint32_t *tab = net->pg_tab + (size_t)cn->src * POISSON_TABLE_MAX;
-        uint32_t ctr[4] = {(uint32_t)j, (uint32_t)T, (uint32_t)((uint64_t)T >> 32),
-                           0x504F4953u};
-        uint32_t out[4];
-        orc_philox4x32_10(ctr, key, out);
-        int n = orc_poisson_sample(tab, out[0]);
+        uint32_t u = orc_pois_word((uint64_t)net->seed, (uint32_t)j, T,
+                                   0x504F4953u);
+        int n = orc_poisson_sample(tab, u);
         net->cnt_pois++;
         if (n > 0)
           acc[(size_t)cn->port * nn + cn->tgt] += (ACC)(cn->weight * (float)n);

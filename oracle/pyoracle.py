@@ -163,6 +163,7 @@ def lib(prec="f32"):
         L.orc_poisson_sample.restype = i
         L.orc_poisson_sample.argtypes = [vp, ctypes.c_uint32]
         L.orc_philox4x32_10.argtypes = [vp, vp, vp]
+        L.orc_threefry2x32_20.argtypes = [vp, vp, vp]
         _LIBS[prec] = L
     return _LIBS[prec]
 
@@ -173,6 +174,15 @@ def philox4x32_10(ctr, key):
     k = np.asarray(key, dtype=np.uint32)
     o = np.zeros(4, dtype=np.uint32)
     L.orc_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
+
+
+def threefry2x32_20(ctr, key):
+    L = lib()
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(2, dtype=np.uint32)
+    L.orc_threefry2x32_20(c.ctypes.data, k.ctypes.data, o.ctypes.data)
     return o
 
 
@@ -299,7 +309,7 @@ class Group:
 class Net:
     """One simulated network in the oracle.
 
-    dt and t0 in ms; seed keys the Poisson Philox streams."""
+    dt and t0 in ms; seed keys the Poisson Threefry streams."""
 
     def __init__(self, dt=0.1, t0=0.0, seed=54321, prec="f32"):
         self.L = lib(prec)

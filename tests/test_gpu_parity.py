@@ -179,7 +179,7 @@ def test_network_brunel_rate(ngpu):
 def test_c0_subthreshold_bit_exact(ngpu):
     """Small configuration exactly as specified (silent: the Poisson drive
     stays below rheobase): every membrane potential after 200 ms equals the
-    FP32 oracle's bit for bit (Philox Poisson streams, fold, RK5)."""
+    FP32 oracle's bit for bit (Threefry Poisson streams, fold, RK5)."""
     log("case c0_literal")
     spec = nb.c0_spec(n=1000, seed=1234)
     groups, _ = nb.build_engine(ngpu, spec)

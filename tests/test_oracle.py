@@ -38,6 +38,20 @@ def test_philox_kat(ctr, key, want):
 
 
 # ---------------------------------------------------------------------
+# Threefry2x32-20 (the Poisson draws): Random123 known-answer vectors
+# (kat_vectors, threefry2x32_20)
+# ---------------------------------------------------------------------
+@pytest.mark.parametrize("ctr,key,want", [
+    ([0, 0], [0, 0], [0x6b200159, 0x99ba4efe]),
+    ([0xffffffff] * 2, [0xffffffff] * 2, [0x1cb996fc, 0xbb002be7]),
+    ([0x243f6a88, 0x85a308d3], [0x13198a2e, 0x03707344],
+     [0xc4923a9c, 0x483df7a0]),
+])
+def test_threefry_kat(ctr, key, want):
+    assert list(po.threefry2x32_20(ctr, key)) == want
+
+
+# ---------------------------------------------------------------------
 # Poisson sampler: table of 32-bit CDF thresholds + inversion
 # ---------------------------------------------------------------------
 @pytest.mark.parametrize("lam", [1e-4, 0.01, 0.1, 0.8, 2.0, 8.0, 40.0])
@@ -75,16 +89,17 @@ def test_poisson_zero_and_too_large_rate():
 
 
 def test_poisson_counts_distribution():
-    """Counts drawn with the Philox counter layout of the direct
-    connections follow Poisson(lambda) (chi-square, p > 1e-4)."""
+    """Counts drawn with the Threefry counter layout of the direct
+    connections (counter (index, step), key (seed ^ tag, seed_hi)) follow
+    Poisson(lambda) (chi-square, p > 1e-4)."""
     L = po.lib()
     lam = 0.8
     t = np.ascontiguousarray(po.poisson_table(lam), dtype=np.uint32)
     n = 200000
     counts = np.zeros(12, dtype=np.int64)
     for i in range(n):
-        u = int(po.philox4x32_10([i % 1000, i // 1000, 0, 0x504F4953],
-                                 [1234, 0])[0])
+        u = int(po.threefry2x32_20([i % 1000, i // 1000],
+                                   [1234 ^ 0x504F4953, 0])[0])
         c = L.orc_poisson_sample(t.ctypes.data, u)
         counts[min(c, 11)] += 1
     pmf = scipy.stats.poisson.pmf(np.arange(11), lam)

@@ -49,6 +49,9 @@ __device__ __forceinline__ void aeif_rhs(const AeifLocal<P> &p, float refr,
                                          float (&dy)[2 + 2 * P]) {
   bool rf = refr > 0.0f;
   float V = rf ? p.V_reset : ((y[0] < p.V_peak) ? y[0] : p.V_peak);
+  // refractory: dV = 0, by a zero factor rather than a branch (a wave never
+  // splits on the refractory state); (t/C_m)*0 and t*0 agree bit for bit
+  const float inv_C_m = rf ? 0.0f : p.inv_C_m;
   float I_syn = 0.0f;
 #pragma unroll
   for (int i = 0; i < P; i++)
@@ -59,9 +62,7 @@ __device__ __forceinline__ void aeif_rhs(const AeifLocal<P> &p, float refr,
   t = t + I_syn;
   t = t - y[1];
   t = t + p.I_e;
-  // refractory: dV = 0, by a 0/1 factor rather than a branch, so that a
-  // wave never splits on the refractory state
-  dy[0] = (t * p.inv_C_m) * (rf ? 0.0f : 1.0f);
+  dy[0] = t * inv_C_m;
   float u = p.a * (V - p.E_L);
   u = u - y[1];
   dy[1] = u * p.inv_tau_w;
@@ -98,8 +99,8 @@ template <int P>
 __device__ __forceinline__ float aeif_rk5_attempt(
     const AeifLocal<P> &p, float refr, const float (&y)[2 + 2 * P],
     const float (&k1)[2 + 2 * P], const float (&scal)[2 + 2 * P],
-    float hh, float (&yn)[2 + 2 * P]) {
-  return rk5_attempt(AeifModel<P>{p}, refr, y, k1, scal, hh, yn);
+    float hh, float eps, float (&yn)[2 + 2 * P]) {
+  return rk5_attempt(AeifModel<P>{p}, refr, y, k1, scal, hh, eps, yn);
 }
 
 template <int P>
@@ -413,8 +414,7 @@ __device__ __forceinline__ bool aeif_attempt_and_update(
   if (hh > h_max) hh = h_max;
   if (hh < A.h_min) hh = A.h_min;
   h = hh;
-  float err = aeif_rk5_attempt<P>(p, refr, y, k1, scal, hh, yn);
-  err = err / A.eps;
+  float err = aeif_rk5_attempt<P>(p, refr, y, k1, scal, hh, A.eps, yn);
   err_out = err;
   hh_out = hh;
   n_att++;

@@ -15,7 +15,8 @@ namespace ngpu {
 
 // exp(x): Cody-Waite reduction x = k ln2 + r (ln2 split 0x3F317200 +
 // 1.42860677e-06), exp(r) by the degree-7 Taylor polynomial on
-// |r| <= ln2/2, then 2^k applied as two exact power-of-two products.
+// |r| <= ln2/2, then 2^k applied by ldexp (one instruction; exact, or
+// rounded once where the result is subnormal).
 // Written without branches or selects: x is clamped to [-104, 88.8] by
 // NaN-propagating minimum/maximum, and the polynomial path already gives
 // +inf at 88.8 and +0 at -104 (and NaN for NaN), the values beyond the
@@ -34,12 +35,7 @@ __device__ __forceinline__ float dev_expf(float x) {
   p = __builtin_fmaf(p, r, 0.5f);
   p = __builtin_fmaf(p, r, 1.0f);
   p = __builtin_fmaf(p, r, 1.0f);
-  int k = (int)kf;
-  int k1 = k / 2;
-  int k2 = k - k1;
-  float s1 = __uint_as_float((uint32_t)(k1 + 127) << 23);
-  float s2 = __uint_as_float((uint32_t)(k2 + 127) << 23);
-  return (p * s1) * s2;
+  return __builtin_amdgcn_ldexpf(p, (int)kf);
 }
 
 // log(x) for x > 0: x = 2^e m with m in [sqrt(1/2), sqrt(2)),

@@ -229,16 +229,15 @@ __device__ __forceinline__ void rk5_begin(const M &m, float refr,
 }
 
 // One attempt of the step with size hh (src/rk5.h:70-135): stages 2-6,
-// the fifth-order solution in yn and the scaled error norm
-// max_i |hh e_i| / scal_i (before the division by the tolerance).  The
-// largest quotient is found by cross-multiplication (|hh e_i| scal_m
-// against |hh e_m| scal_i, a NaN taking the place, as a NaN quotient
-// does in the reference's running maximum), so that only the winner is
-// divided.
+// the fifth-order solution in yn and the error norm relative to the
+// tolerance, max_i |hh e_i| / (scal_i eps).  The largest quotient is found
+// by cross-multiplication (|hh e_i| scal_m against |hh e_m| scal_i, a NaN
+// taking the place, as a NaN quotient does in the reference's running
+// maximum), so that only the winner is divided, once.
 template <class M>
 __device__ __forceinline__ float rk5_attempt(
     const M &m, float refr, const float (&y)[M::N], const float (&k1)[M::N],
-    const float (&scal)[M::N], float hh, float (&yn)[M::N]) {
+    const float (&scal)[M::N], float hh, float eps, float (&yn)[M::N]) {
   constexpr int N = M::N;
   float k2[N], k3[N], k4[N], k5[N], k6[N];
 #pragma unroll
@@ -302,7 +301,7 @@ __device__ __forceinline__ float rk5_attempt(
     num = take ? v : num;
     den = take ? scal[i] : den;
   }
-  return num / den;
+  return num / (den * eps);
 }
 
 // An attempt accepted with a scaled error above 1, which the acceptance
@@ -329,8 +328,7 @@ __device__ __forceinline__ bool rk5_step(const M &m, float refr, double &x,
     if (hh > h_max) hh = h_max;
     if (hh < h_min) hh = h_min;
     h = hh;
-    err = rk5_attempt(m, refr, y, k1, scal, hh, yn);
-    err = err / eps;
+    err = rk5_attempt(m, refr, y, k1, scal, hh, eps, yn);
     n_att++;
     if (err <= 1.0f || (double)hh <= h_min_acc) break;
     n_rej++;

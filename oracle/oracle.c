@@ -145,8 +145,8 @@ int orc_poisson_sample(const uint32_t *tab, uint32_t u) {
 /* ------------------------------------------------------------------ */
 /* FP32 exp and log (own implementation, mirrored in the HIP kernels).  */
 /* exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, exp(r) by its  */
-/* degree-7 Taylor polynomial (Horner, fma), scaling by 2^k in two exact */
-/* power-of-two multiplications.                                        */
+/* degree-7 Taylor polynomial (Horner, fma), scaling by 2^k with ldexp    */
+/* (exact, or rounded once where the result is subnormal).              */
 /* ------------------------------------------------------------------ */
 static inline float orc_u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static inline uint32_t orc_f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
@@ -166,12 +166,7 @@ float orc_expf(float x) {
   p = fmaf(p, r, 0.5f);
   p = fmaf(p, r, 1.0f);
   p = fmaf(p, r, 1.0f);
-  int k = (int)kf;
-  int k1 = k / 2;
-  int k2 = k - k1;
-  float s1 = orc_u2f((uint32_t)(k1 + 127) << 23);
-  float s2 = orc_u2f((uint32_t)(k2 + 127) << 23);
-  return (p * s1) * s2;
+  return ldexpf(p, (int)kf);
 }
 
 /* log: x = 2^e m, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
@@ -360,8 +355,7 @@ static int orc_rk5_step(int nvar, double *x, REAL *y, REAL *h, REAL h_min,
         den = scal[i];
       }
     }
-    err = num / den;
-    err = err / eps;
+    err = num / (den * eps);
     cnt->n_att++;
     if (err <= (REAL)1 || (double)hh <= h_min_acc) break;
     cnt->n_rej++;
@@ -449,8 +443,8 @@ static void aeif_rhs(int nvar, const REAL *y, REAL *dy, const void *ctx) {
   t = t + I_syn;
   t = t - y[1];
   t = t + p->I_e;
-  /* refractory: dV = t / C_m times 0 (the kernels' form) */
-  dy[0] = (t * p->inv_C_m) * (refr ? (REAL)0 : (REAL)1);
+  /* refractory: dV = t times a zero 1/C_m (the kernels' form) */
+  dy[0] = t * (refr ? (REAL)0 : p->inv_C_m);
   REAL u = p->a * (V - p->E_L);
   u = u - y[1];
   dy[1] = u * p->inv_tau_w;

@@ -565,3 +565,47 @@ def test_launch_settings(ngpu):
                    ("device_connect", 2), ("delivery_tile_fixed", 2)):
         with pytest.raises(ValueError):
             ngpu.SetKernelStatus(k, bad)
+
+
+def test_autotuner_keeps_incumbent_within_noise():
+    """The autotuner replaces a setting only when the candidate's median
+    window is more than the margin faster; a fake engine whose windows
+    scatter by up to +-1 % around the same mean keeps every default, and
+    one where a value is 10 % faster takes it."""
+    import random
+    from neurongpu_amd import autotune as at
+
+    class Fake:
+        def __init__(self, fast=None):
+            self.s = {k: v[0] for k, v in at.CANDIDATES.items()}
+            self.fast = fast or {}
+            self.rng = random.Random(7)
+
+        def GetKernelStatus(self, k):
+            return self.s[k]
+
+        def SetKernelStatus(self, k, v):
+            self.s[k] = v
+
+        def GetTimeResolution(self):
+            return 0.1
+
+        def Simulate(self, t):
+            f = 1.0 + self.rng.uniform(-0.01, 0.01)
+            for k, v in self.fast.items():
+                if self.s[k] == v:
+                    f *= 0.9
+            time_ = 1e-4 * f * t / 0.1
+            end = at.time.perf_counter() + time_
+            while at.time.perf_counter() < end:
+                pass
+
+    eng = Fake()
+    out = at.autotune(eng, steps=4, reps=3)
+    for k, v in at.CANDIDATES.items():
+        assert out[k] == v[0], k
+    eng = Fake(fast={"delivery_grid": 4096})
+    out = at.autotune(eng, steps=4, reps=3)
+    assert out["delivery_grid"] == 4096
+    assert at.choose([1.0, 1.0, 1.0], [0.99, 0.99, 0.99], 0.02) is False
+    assert at.choose([1.0, 1.0, 1.0], [0.9, 0.9, 2.0], 0.02) is True

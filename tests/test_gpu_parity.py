@@ -884,13 +884,17 @@ def test_ext_neuron_bit_exact(ngpu):
 
 @pytest.mark.parametrize("launch", [{"delivery_mode": 0},
                                     {"delivery_mode": 1},
+                                    {"delivery_mode": 3},
+                                    {"delivery_mode": 3,
+                                     "delivery_tile_fixed": 1},
                                     {"graph_steps": 16}])
 def test_legacy_poisson_network_bit_exact(ngpu, launch):
     """Network of the reference's c++/examples/test_error.cpp (1000
     aeif_cond_beta neurons, CreatePoissonGenerator at 20 kHz, one node per
     neuron), dyadic weights, 100 ms: the neurons' and the generators' spike
-    trains equal the FP32 oracle's spike for spike.  Generator spikes carry
-    the time of the step before (src/neurongpu.cu:461-470)."""
+    trains equal the FP32 oracle's spike for spike, with atomic, binned
+    and tiled delivery (generator spikes have heights above 1).  Generator
+    spikes carry the time of the step before (src/neurongpu.cu:461-470)."""
     log("case legacy_poisson_network", launch)
     spec = nb.legacy_brunel_spec()
     steps, n = 1000, 1000

@@ -361,7 +361,7 @@ class Engine {
   float tiled_max_w = 0.0f;         // largest |weight|
   int tiled_fixed = 0;              // fixed-point tile sums (where they fit)
   int delivery_tile_split = 0;
-  int delivery_tile_shift = 14;   // log2 of the accumulator entries per tile
+  int delivery_tile_shift = 15;   // log2 of the accumulator entries per tile
   bool PrepareTiled(TileArgs &t);
   // the tiled delivery visits every (spike, delay) pair once per tile: it
   // pays where the store's (source, delay, tile) segments average at least

@@ -293,7 +293,8 @@ def test_network_statistics_vs_fp64(ngpu):
     {"delivery_mode": 2, "delivery_auto_events": 100000},
     {"delivery_mode": 3},
     {"delivery_mode": 3, "delivery_tile_split": 3},
-    {"delivery_mode": 3, "delivery_tile_fixed": 1},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1,
+     "delivery_tile_shift": 14},
 ])
 def test_c2_iaf_network_bit_exact(ngpu, settings):
     """c2-shaped iaf_psc_exp network at 10 % size (7717 neurons, 5e6
@@ -412,8 +413,10 @@ def test_ensemble_routing_matches_instances_alone(ngpu):
     {"delivery_mode": 3, "delivery_tile_split": 7, "delivery_tile_max": 2},
     {"delivery_mode": 3, "delivery_tile_shift": 10},
     {"delivery_mode": 3, "delivery_tile_shift": 11, "delivery_tile_split": 1},
-    {"delivery_mode": 3, "delivery_tile_fixed": 1},
-    {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_split": 3},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1,
+     "delivery_tile_shift": 14},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_split": 3,
+     "delivery_tile_shift": 14},
 ])
 def test_launch_shapes_do_not_change_results(ngpu, settings):
     """The autotuner's choices (RK5 two-pass budget and grid, delivery grid,
@@ -769,8 +772,10 @@ def test_point_model_network_bit_exact(ngpu, model):
 
 @pytest.mark.parametrize("settings", [
     {"delivery_mode": 3},
-    {"delivery_mode": 3, "delivery_tile_fixed": 1},
-    {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_split": 3},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1,
+     "delivery_tile_shift": 14},
+    {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_split": 3,
+     "delivery_tile_shift": 14},
     {"delivery_mode": 3, "delivery_tile_fixed": 1, "delivery_tile_shift": 10},
 ])
 def test_tiled_delivery_spike_heights(ngpu, settings):
@@ -834,7 +839,8 @@ def test_tiled_fixed_point_sums_are_deterministic(ngpu):
         return (np.asarray(ngpu.GetNeuronVar(0, n, "V_m"), np.float32),
                 np.asarray(ngpu.GetNeuronVar(0, n, "g1"), np.float32))
 
-    fx = {"delivery_mode": 3, "delivery_tile_fixed": 1}
+    fx = {"delivery_mode": 3, "delivery_tile_fixed": 1,
+          "delivery_tile_shift": 14}
     a = run(dict(fx, delivery_tile_split=1))
     b = run(dict(fx, delivery_tile_split=5))
     c = run(dict(fx, delivery_tile_split=5))
@@ -886,7 +892,8 @@ def test_ext_neuron_bit_exact(ngpu):
                                     {"delivery_mode": 1},
                                     {"delivery_mode": 3},
                                     {"delivery_mode": 3,
-                                     "delivery_tile_fixed": 1},
+                                     "delivery_tile_fixed": 1,
+                                     "delivery_tile_shift": 14},
                                     {"graph_steps": 16}])
 def test_legacy_poisson_network_bit_exact(ngpu, launch):
     """Network of the reference's c++/examples/test_error.cpp (1000

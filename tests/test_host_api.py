@@ -542,7 +542,7 @@ def test_launch_settings(ngpu):
                 "delivery_auto_events": 500000, "graph_steps": 0,
                 "delivery_streams": 2, "step_overlap": 2,
                 "device_connect": 1, "delivery_tile_split": 0,
-                "delivery_tile_shift": 14, "delivery_tile_fixed": 0}
+                "delivery_tile_shift": 15, "delivery_tile_fixed": 0}
     for k, v in defaults.items():
         assert ngpu.GetKernelStatus(k) == v
     ngpu.SetKernelStatus("delivery_mode", 1)

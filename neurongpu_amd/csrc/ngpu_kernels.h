@@ -433,6 +433,7 @@ struct TileArgs {
   float q_scale, q_inv, h_max;
 };
 constexpr float kTileHeightMax = 1024.0f;
+constexpr int kTiledPairTile = 32;  // most pairs per wave tile, tiled mode
 
 struct RevArgs {
   const uint32_t *ring_src;

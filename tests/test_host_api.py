@@ -556,7 +556,7 @@ def test_launch_settings(ngpu):
     ngpu.SetKernelStatus("delivery_tile_split", 8)
     assert ngpu.GetKernelStatus("delivery_tile_split") == 8
     for k, bad in (("delivery_mode", 4), ("delivery_tile_max", 3),
-                   ("delivery_tile_split", -1), ("delivery_tile_split", 65),
+                   ("delivery_tile_split", -1), ("delivery_tile_split", 257),
                    ("delivery_tile_shift", 9), ("delivery_tile_shift", 16),
                    ("delivery_grid", 0), ("rk5_budget", 0),
                    ("delivery_bin_capacity", 0), ("graph_steps", -1),

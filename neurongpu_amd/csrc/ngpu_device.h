@@ -1,5 +1,6 @@
 // Device helpers of the per-timestep kernels: FP32 exp/log, the
-// Cash-Karp RK5 tableau, Philox4x32-10 and the synapse update rules.
+// Cash-Karp RK5 tableau, Threefry2x32-20 and Philox4x32-10, and the
+// synapse update rules.
 //
 // Every multiply-add that is fused is written as an explicit fmaf; the
 // kernels are compiled with -ffp-contract=off, so the FP32 arithmetic is a

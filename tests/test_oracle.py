@@ -2,7 +2,7 @@
 
 Each oracle function is checked against something it does not compute
 itself: the reference's golden vectors (NEST traces, STDP scenarios,
-spike-detector timing), published known-answer tests (Philox), library
+spike-detector timing), published known-answer tests (Philox, Threefry), library
 routines (scipy's Poisson CDF, matrix exponential, root finder), closed
 forms (beta-peak normalisation, STDP rule, AdEx fixed point) and
 convergence order.  Runs on the CPU only."""

@@ -1302,7 +1302,9 @@ __global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
 //              atomics instead and the later phases do nothing.
 //   4 reduce   one block per chunk of events of one bin sums them in LDS
 //              (chunk size chosen from the step's event count, see
-//              bin_chunk_size); a bin of one chunk is added to the
+//              bin_chunk_size, or one chunk per bin when there are at
+//              least kSingleChunkBins bins of about equal size; then the
+//              sum phase has nothing to do); a bin of one chunk is added to the
 //              accumulators directly, otherwise the partial sums are
 //              written out
 //   5 sum      bins of several chunks: accumulators += sum of partials.
@@ -1409,6 +1411,8 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
   uint32_t *s_cur = s_dyn + A.D + 1;
   __shared__ uint32_t s_part[kBlock];
   __shared__ unsigned long long s_part64[kBlock];
+  __shared__ int s_multi;  // some bin has more than one chunk
+  if (threadIdx.x == 0) s_multi = 0;
   const unsigned long long n_ev = block_scan_excl(
       B.nsub, [&](int b) -> unsigned long long { return B.tot[b]; }, s_cur,
       s_part64);
@@ -1418,11 +1422,35 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
     // gives about kReduceChunks chunks in all (at least kBinChunkMin
     // events each), so the reduction fills the device whatever the step's
     // event count
-    const uint32_t ch = bin_chunk_size(n_ev);
+    uint32_t ch = bin_chunk_size(n_ev);
     const int spb = kBinLen >> B.sub_shift;  // sub-bins per bin
     auto start = [&](int b) -> unsigned long long {
       return b * spb < B.nsub ? s_cur[b * spb] : n_ev;
     };
+    // With many bins of about equal size, one chunk per bin: the reduction
+    // still has a block for most CUs, and no partial sums are written and
+    // summed.
+    if (B.nb >= kSingleChunkBins) {
+      unsigned long long mx = 0;
+      for (int b = threadIdx.x; b < B.nb; b += kBlock) {
+        const unsigned long long nbin = start(b + 1) - start(b);
+        mx = nbin > mx ? nbin : mx;
+      }
+      s_part64[threadIdx.x] = mx;
+      __syncthreads();
+      for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off &&
+            s_part64[threadIdx.x + off] > s_part64[threadIdx.x])
+          s_part64[threadIdx.x] = s_part64[threadIdx.x + off];
+        __syncthreads();
+      }
+      mx = s_part64[0];
+      __syncthreads();
+      if (mx * (unsigned long long)B.nb <= 2ull * n_ev) {
+        const unsigned long long one = (mx + 1023ull) & ~1023ull;
+        if (one > ch && one <= 0x40000000ull) ch = (uint32_t)one;
+      }
+    }
     for (int b = threadIdx.x; b < B.nb; b += kBlock)
       B.bstart[b] = binned ? (uint32_t)start(b) : 0u;
     if (threadIdx.x == 0) {
@@ -1432,10 +1460,16 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
     unsigned long long nc = block_scan_excl(
         B.nb,
         [&](int b) -> unsigned long long {
-          return binned ? (start(b + 1) - start(b) + ch - 1u) / ch : 0u;
+          const unsigned long long c =
+              binned ? (start(b + 1) - start(b) + ch - 1u) / ch : 0u;
+          if (c > 1u) s_multi = 1;
+          return c;
         },
         B.cstart, s_part64);
-    if (threadIdx.x == 0) B.cstart[B.nb] = (uint32_t)nc;
+    if (threadIdx.x == 0) {
+      B.cstart[B.nb] = (uint32_t)nc;
+      B.meta[1] = (uint32_t)s_multi;
+    }
   }
   for (int b = threadIdx.x; b < B.nsub; b += kBlock)
     s_cur[b] += B.hist[(size_t)b * B.grid + blockIdx.x];
@@ -1518,6 +1552,7 @@ __global__ __launch_bounds__(kReduceBlock) void bin_reduce_kernel(BinArgs B,
 __global__ __launch_bounds__(kBlock) void bin_sum_kernel(
     BinArgs B, float *acc, const StepClock *clock, StepClock *clock_next) {
   clock_step(clock, clock_next);
+  if (B.meta[1] == 0u) return;  // every bin was summed by one chunk
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
        i < B.acc_n; i += (long long)gridDim.x * kBlock) {
     const int b = (int)(i >> kBinShift);

@@ -368,6 +368,8 @@ constexpr int kMaxBins = 8192;            // sub-bins a block can count in LDS
 constexpr int kSubBinsTarget = 64;        // preferred largest sub-bin count
 constexpr int kReduceChunks = 512;        // target chunk count of a step
 constexpr int kBinChunkMin = 4096;        // fewest events per chunk
+constexpr int kSingleChunkBins = 128;     // from this many bins on, balanced
+                                          // bins are reduced in one chunk each
 
 struct BinArgs {
   // The count and scatter phases group events by sub-bin (2^sub_shift
@@ -382,7 +384,8 @@ struct BinArgs {
   unsigned long long *ev;   // [ev_cap] (index within bin, value bits)
   float *part;              // [chunk_cap][kBinLen] partial sums,
                             // chunk_cap = kReduceChunks + nb
-  uint32_t *meta;           // [1] chunk size of the step
+  uint32_t *meta;           // [2] chunk size of the step, and whether
+                            // some bin has more than one chunk
   long long ev_cap;         // above this many events the step falls back
                             // to atomic delivery
   int nb;                   // bins: ceil(acc_n / kBinLen)

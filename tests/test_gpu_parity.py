@@ -323,6 +323,56 @@ def test_c2_iaf_network_bit_exact(ngpu, settings):
     assert eng == orc
 
 
+def _wide_iaf_spec(n=540000, k_in=10, seed=11):
+    """n iaf_psc_exp neurons (80 % excitatory), k_in uniformly drawn sources
+    each, whole-pA weights (+88 pA on port 0 from E, -352 pA on port 1 from
+    I), delays 1.0-2.0 ms in 0.1 ms steps, one 16 kHz Poisson generator at
+    88 pA on every neuron.  2 ports x 540000 nodes = 132 accumulator bins of
+    8192 entries."""
+    rng = np.random.default_rng(seed)
+    ne = (4 * n) // 5
+    tgt = np.repeat(np.arange(n, dtype=np.int32), k_in)
+    src = rng.integers(0, n, n * k_in).astype(np.int32)
+    d = (10 + rng.integers(0, 11, n * k_in)) * 0.1
+    exc = src < ne
+    conns = [
+        {"src": src[exc], "tgt": tgt[exc], "weight": np.float32(88.0),
+         "delay": d[exc], "port": 0},
+        {"src": src[~exc], "tgt": tgt[~exc], "weight": np.float32(-352.0),
+         "delay": d[~exc], "port": 1},
+        {"src": np.full(n, n, np.int32), "tgt": np.arange(n, dtype=np.int32),
+         "weight": np.float32(88.0), "delay": np.float32(0.1), "port": 0},
+    ]
+    return {"dt": 0.1, "seed": seed,
+            "groups": [{"model": "iaf_psc_exp", "n": n,
+                        "vars": {"V_m_rel": rng.uniform(0.0, 15.0, n)
+                                 .astype(np.float32)}},
+                       {"model": "poisson_generator", "n": 1}],
+            "poisson": [{"node": n, "rate": 16000.0}],
+            "conns": conns}
+
+
+def test_binned_delivery_one_chunk_per_bin(ngpu):
+    """Binned delivery with more than 128 accumulator bins of about equal
+    load, where each bin is reduced by one block and no partial sums are
+    written: 540000 iaf_psc_exp neurons, 5.4e6 synapses with whole-pA
+    weights, spike trains equal to the FP32 oracle's over 20 ms."""
+    log("case binned_one_chunk_per_bin")
+    spec = _wide_iaf_spec()
+    n = spec["groups"][0]["n"]
+    groups, _ = nb.build_engine(ngpu, spec)
+    ngpu.SetKernelStatus("delivery_mode", 1)
+    ngpu.ActivateRecSpikeTimes(groups[0], 50)
+    ngpu.Simulate(20.0)
+    eng = sorted(nb.engine_spikes(ngpu, groups[0]))
+    net, _ = nb.build_oracle(spec, "f32")
+    net.run(200, 16)
+    orc = sorted(nb.oracle_spikes(net, 0.1, 0, n))
+    log("spikes engine", len(eng), "oracle", len(orc))
+    assert len(orc) > 10000
+    assert eng == orc
+
+
 def test_c3_stdp_network_statistics(ngpu):
     """c3-shaped plastic network at 2 % size (4000 aeif_cond_alpha neurons,
     2e6 synapses, STDP on E -> E), 500 ms: population rates within 2 % of

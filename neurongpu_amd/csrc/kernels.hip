@@ -1313,6 +1313,7 @@ __global__ __launch_bounds__(kBlock) void delivery_kernel(DeliveryArgs A) {
 // ---------------------------------------------------------------------
 
 constexpr int kReduceBlock = 1024;
+constexpr int kBinBlock = 512;  // threads per block of the count and scatter
 
 // Float add into LDS by compare-and-swap.  On gfx950 the LDS float atomic
 // add runs at about a twelfth of the rate of the integer atomics and of a
@@ -1339,11 +1340,11 @@ __device__ __forceinline__ uint32_t bin_chunk_size(unsigned long long n_ev) {
 
 // Exclusive scan of val(0..n-1) by one block: out[b] = sum of val(b') for
 // b' < b.  Returns the 64-bit total.  s_part holds kBlock entries.
-template <class F>
+template <int BS = kBlock, class F>
 __device__ __forceinline__ unsigned long long block_scan_excl(
     int n, F val, uint32_t *out, unsigned long long *s_part) {
   const int tid = threadIdx.x;
-  const int per = (n + kBlock - 1) / kBlock;
+  const int per = (n + BS - 1) / BS;
   unsigned long long run = 0;
   for (int k = 0; k < per; k++) {
     int b = tid * per + k;
@@ -1351,7 +1352,7 @@ __device__ __forceinline__ unsigned long long block_scan_excl(
   }
   s_part[tid] = run;
   __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
+  for (int off = 1; off < BS; off <<= 1) {
     unsigned long long v = (tid >= off) ? s_part[tid - off] : 0ull;
     __syncthreads();
     s_part[tid] += v;
@@ -1366,20 +1367,21 @@ __device__ __forceinline__ unsigned long long block_scan_excl(
       acc += v;
     }
   }
-  unsigned long long total = s_part[kBlock - 1];
+  unsigned long long total = s_part[BS - 1];
   __syncthreads();
   return total;
 }
 
-__global__ __launch_bounds__(kBlock) void bin_count_kernel(DeliveryArgs A0,
-                                                           BinArgs B) {
+__global__ __launch_bounds__(kBinBlock) void bin_count_kernel(DeliveryArgs A0,
+                                                              BinArgs B) {
   const DeliveryArgs A = clocked(A0);
   uint32_t *s_pre = s_dyn;
   uint32_t *s_h = s_dyn + A.D + 1;
-  __shared__ uint32_t s_part[kBlock];
-  for (int b = threadIdx.x; b < B.nsub; b += kBlock) s_h[b] = 0u;
-  const uint32_t total = delivery_prologue(A, blockIdx.x, s_pre, s_part);
-  deliver_events<4>(A, blockIdx.x, gridDim.x, s_pre, total,
+  __shared__ uint32_t s_part[kBinBlock];
+  for (int b = threadIdx.x; b < B.nsub; b += kBinBlock) s_h[b] = 0u;
+  const uint32_t total =
+      delivery_prologue<kBinBlock>(A, blockIdx.x, s_pre, s_part);
+  deliver_events<4, kBinBlock>(A, blockIdx.x, gridDim.x, s_pre, total,
                  [&](bool ok, unsigned long long, unsigned long long r, float,
                      uint32_t) {
                    if (!ok) return;
@@ -1389,7 +1391,7 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(DeliveryArgs A0,
                    atomicAdd(&s_h[idx >> B.sub_shift], 1u);
                  });
   __syncthreads();
-  for (int b = threadIdx.x; b < B.nsub; b += kBlock)
+  for (int b = threadIdx.x; b < B.nsub; b += kBinBlock)
     B.hist[(size_t)b * B.grid + blockIdx.x] = s_h[b];
 }
 
@@ -1404,16 +1406,16 @@ __global__ __launch_bounds__(kBlock) void bin_scan_kernel(BinArgs B) {
     B.tot[blockIdx.x] = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
 }
 
-__global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
-                                                             BinArgs B) {
+__global__ __launch_bounds__(kBinBlock) void bin_scatter_kernel(
+    DeliveryArgs A0, BinArgs B) {
   const DeliveryArgs A = clocked(A0);
   uint32_t *s_pre = s_dyn;
   uint32_t *s_cur = s_dyn + A.D + 1;
-  __shared__ uint32_t s_part[kBlock];
-  __shared__ unsigned long long s_part64[kBlock];
+  __shared__ uint32_t s_part[kBinBlock];
+  __shared__ unsigned long long s_part64[kBinBlock];
   __shared__ int s_multi;  // some bin has more than one chunk
   if (threadIdx.x == 0) s_multi = 0;
-  const unsigned long long n_ev = block_scan_excl(
+  const unsigned long long n_ev = block_scan_excl<kBinBlock>(
       B.nsub, [&](int b) -> unsigned long long { return B.tot[b]; }, s_cur,
       s_part64);
   const bool binned = n_ev <= (unsigned long long)B.ev_cap;
@@ -1432,13 +1434,13 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
     // summed.
     if (B.nb >= kSingleChunkBins) {
       unsigned long long mx = 0;
-      for (int b = threadIdx.x; b < B.nb; b += kBlock) {
+      for (int b = threadIdx.x; b < B.nb; b += kBinBlock) {
         const unsigned long long nbin = start(b + 1) - start(b);
         mx = nbin > mx ? nbin : mx;
       }
       s_part64[threadIdx.x] = mx;
       __syncthreads();
-      for (int off = kBlock / 2; off > 0; off >>= 1) {
+      for (int off = kBinBlock / 2; off > 0; off >>= 1) {
         if (threadIdx.x < off &&
             s_part64[threadIdx.x + off] > s_part64[threadIdx.x])
           s_part64[threadIdx.x] = s_part64[threadIdx.x + off];
@@ -1451,13 +1453,13 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
         if (one > ch && one <= 0x40000000ull) ch = (uint32_t)one;
       }
     }
-    for (int b = threadIdx.x; b < B.nb; b += kBlock)
+    for (int b = threadIdx.x; b < B.nb; b += kBinBlock)
       B.bstart[b] = binned ? (uint32_t)start(b) : 0u;
     if (threadIdx.x == 0) {
       B.bstart[B.nb] = binned ? (uint32_t)n_ev : 0u;
       B.meta[0] = ch;
     }
-    unsigned long long nc = block_scan_excl(
+    unsigned long long nc = block_scan_excl<kBinBlock>(
         B.nb,
         [&](int b) -> unsigned long long {
           const unsigned long long c =
@@ -1471,11 +1473,12 @@ __global__ __launch_bounds__(kBlock) void bin_scatter_kernel(DeliveryArgs A0,
       B.meta[1] = (uint32_t)s_multi;
     }
   }
-  for (int b = threadIdx.x; b < B.nsub; b += kBlock)
+  for (int b = threadIdx.x; b < B.nsub; b += kBinBlock)
     s_cur[b] += B.hist[(size_t)b * B.grid + blockIdx.x];
   // (the prologue's barriers order these LDS writes before their use)
-  const uint32_t total = delivery_prologue(A, blockIdx.x, s_pre, s_part);
-  unsigned long long n_events = deliver_events<4>(
+  const uint32_t total =
+      delivery_prologue<kBinBlock>(A, blockIdx.x, s_pre, s_part);
+  unsigned long long n_events = deliver_events<4, kBinBlock>(
       A, blockIdx.x, gridDim.x, s_pre, total,
       [&](bool ok, unsigned long long, unsigned long long r, float hj,
           uint32_t) {
@@ -2060,11 +2063,11 @@ void LaunchDeliveryBinned(const DeliveryArgs &a, const BinArgs &b,
                           int reduce_grid, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   const size_t lds = delivery_lds_bytes(a.D, b.nsub);
-  hipLaunchKernelGGL(bin_count_kernel, dim3(b.grid), dim3(kBlock), lds, s, a,
-                     b);
+  hipLaunchKernelGGL(bin_count_kernel, dim3(b.grid), dim3(kBinBlock), lds, s,
+                     a, b);
   hipLaunchKernelGGL(bin_scan_kernel, dim3(b.nsub), dim3(kBlock), 0, s, b);
-  hipLaunchKernelGGL(bin_scatter_kernel, dim3(b.grid), dim3(kBlock), lds, s, a,
-                     b);
+  hipLaunchKernelGGL(bin_scatter_kernel, dim3(b.grid), dim3(kBinBlock), lds, s,
+                     a, b);
   hipLaunchKernelGGL(bin_reduce_kernel, dim3(reduce_grid), dim3(kReduceBlock),
                      0, s, b, a.acc);
   long long g = (b.acc_n + kBlock - 1) / kBlock;

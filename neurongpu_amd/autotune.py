@@ -31,7 +31,7 @@ CANDIDATES = {
     "delivery_grid": [1024, 2048, 4096],
     "delivery_tile_max": [8, 16, 64],
     "fuse_delivery": [1, 0],
-    "step_overlap": [2, 0, 1, 4],
+    "step_overlap": [8, 0, 2, 4, 16],
     "graph_steps": [0, 16],
 }
 

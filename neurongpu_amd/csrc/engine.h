@@ -296,7 +296,7 @@ class Engine {
   // second pass, for at most k steps in a row; then a first pass waits
   // for the second pass before it and takes every neuron again.
   // 0: the second pass ends each step.
-  int step_overlap = 2;
+  int step_overlap = 8;
   bool ov_async_prev = false;   // the last step's second pass may still run
   bool ov_skip = false;         // this step's first pass skips its neurons
   bool ov_mask = false;         // this step's second pass may overlap

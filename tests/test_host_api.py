@@ -540,7 +540,7 @@ def test_launch_settings(ngpu):
                 "delivery_tile_max": 64, "delivery_mode": 2,
                 "delivery_bin_capacity": 1 << 27,
                 "delivery_auto_events": 500000, "graph_steps": 0,
-                "delivery_streams": 2, "step_overlap": 2,
+                "delivery_streams": 2, "step_overlap": 8,
                 "device_connect": 1, "delivery_tile_split": 0,
                 "delivery_tile_shift": 15, "delivery_tile_fixed": 0}
     for k, v in defaults.items():

@@ -418,12 +418,15 @@ __device__ __forceinline__ bool aeif_attempt_and_update(
   err_out = err;
   hh_out = hh;
   n_att++;
-  if (err <= 1.0f || (double)hh <= A.h_min_acc) {
+  const bool accept = err <= 1.0f || (double)hh <= A.h_min_acc;
+  // err^-1/5 (accepted) or err^-1/4 (rejected), one sequence for the wave
+  const float pw = dev_pow_ctrl(err, accept ? -0.2f : -0.25f);
+  if (accept) {
     n_acc++;
     if (err > 1.0f) count_forced(A.forced);
     x += (double)h;
     if (err > CK::err_min) {
-      h = (h * CK::coeff) * dev_pow_inc(err);
+      h = (h * CK::coeff) * pw;
     } else {
       h = 5.0f * h;
     }
@@ -434,7 +437,7 @@ __device__ __forceinline__ bool aeif_attempt_and_update(
     if (n_att > kMaxAttemptsPerStep) {
       x = x1;  // give up on this step
     } else {
-      float h_new = (hh * CK::coeff) * dev_pow_dec(err);
+      float h_new = (hh * CK::coeff) * pw;
       float h_lo = 0.1f * hh;
       h = (h_new > h_lo) ? h_new : h_lo;
       return false;  // retry with the smaller step

@@ -333,7 +333,7 @@ __device__ __forceinline__ bool rk5_step(const M &m, float refr, double &x,
     if (err <= 1.0f || (double)hh <= h_min_acc) break;
     n_rej++;
     if (n_att > kMaxAttemptsPerStep) return false;
-    float h_new = (hh * CK::coeff) * dev_pow_dec(err);
+    float h_new = (hh * CK::coeff) * dev_pow_ctrl(err, -0.25f);
     float h_lo = 0.1f * hh;
     h = (h_new > h_lo) ? h_new : h_lo;
   }
@@ -341,7 +341,7 @@ __device__ __forceinline__ bool rk5_step(const M &m, float refr, double &x,
   if (err > 1.0f) count_forced(forced);
   x += (double)h;
   if (err > CK::err_min) {
-    h = (h * CK::coeff) * dev_pow_inc(err);
+    h = (h * CK::coeff) * dev_pow_ctrl(err, -0.2f);
   } else {
     h = 5.0f * h;
   }

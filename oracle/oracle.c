@@ -29,9 +29,9 @@ typedef double REAL;
 typedef float REAL;
 #define FMA(a, b, c) fmaf((a), (b), (c))
 #define EXPR(x) orc_expf(x)
-/* err^(-1/4) = 1/sqrt(sqrt(err)); err^(-1/5) = exp(-0.2*log(err)) */
-#define POW_DEC(e) (1.0f / sqrtf(sqrtf(e)))
-#define POW_INC(e) orc_expf(-0.2f * orc_logf(e))
+/* err^(-1/4) and err^(-1/5) by one polynomial sequence (orc_pow_ctrl) */
+#define POW_DEC(e) orc_pow_ctrl((e), -0.25f)
+#define POW_INC(e) orc_pow_ctrl((e), -0.2f)
 #define ROUNDR(x) roundf(x)
 #define SQRTR(x) sqrtf(x)
 #endif
@@ -190,6 +190,40 @@ float orc_logf(float x) {
   float lm = fmaf(two_s * s2, q, two_s);
   float ef = (float)e;
   return fmaf(ef, 0.693145751953125f, fmaf(ef, 1.42860677e-06f, lm));
+}
+
+/* err^c of the step-size controller (src/rk5.h:122-146: pow(err, -0.20)
+ * after an accepted attempt, pow(err, -0.25) after a rejected one), with no
+ * division: err = 2^k m, m in [sqrt(1/2), sqrt(2)); log2 m = t Q(t),
+ * t = m - 1; Q of degree 7, a least-squares fit of log2(1+t)/t with float
+ * coefficients; y = c k + c log2 m; 2^y = 2^j R(y - j), j = rint(y), R of
+ * degree 5 fitted to 2^f on [-1/2, 1/2].  +inf -> 0, NaN -> NaN. */
+float orc_pow_ctrl(float e, float c) {
+  if (e != e) return e;
+  if (e == orc_u2f(0x7F800000u)) return 0.0f;
+  uint32_t b = orc_f2u(e);
+  int k = (int)((b >> 23) & 0xFFu) - 127;
+  float m = orc_u2f((b & 0x007FFFFFu) | 0x3F800000u);
+  if (m > 1.41421354f) { m = m * 0.5f; k = k + 1; }
+  float t = m - 1.0f;
+  float q = -0.14237648f;
+  q = fmaf(q, t, 0.23247829f);
+  q = fmaf(q, t, -0.24932154f);
+  q = fmaf(q, t, 0.28731275f);
+  q = fmaf(q, t, -0.36022335f);
+  q = fmaf(q, t, 0.48091587f);
+  q = fmaf(q, t, -0.72135293f);
+  q = fmaf(q, t, 1.442695f);
+  float y = fmaf(c, (float)k, c * (t * q));
+  float j = rintf(y);
+  float f = y - j;
+  float r = 0.0013266971f;
+  r = fmaf(r, f, 0.00967546f);
+  r = fmaf(r, f, 0.055507425f);
+  r = fmaf(r, f, 0.24022122f);
+  r = fmaf(r, f, 0.69314694f);
+  r = fmaf(r, f, 1.0000001f);
+  return ldexpf(r, (int)j);
 }
 
 /* STDP rule in double, src/stdp.cu:24-51.  param7 = tau_plus, tau_minus,

@@ -40,9 +40,11 @@
  *     once at calibration and multiplied (the reference divides);
  *   - exp/log in FP32 are this project's own Cody-Waite + polynomial
  *     functions (orc_expf/orc_logf), so GPU and CPU agree bit for bit;
- *   - Poisson counts come from Philox4x32-10 (Salmon et al., SC'11) and
+ *   - Poisson counts come from Threefry2x32-20 (Salmon et al., SC'11) and
  *     inversion by sequential search over a per-rate CDF table of 32-bit
- *     thresholds, instead of curand (closed source).
+ *     thresholds, instead of curand (closed source);
+ *   - the step-size controller's err^-0.2 and err^-0.25 come from one
+ *     polynomial sequence (orc_pow_ctrl) instead of powf.
  */
 #ifndef NGPU_ORACLE_H
 #define NGPU_ORACLE_H
@@ -81,6 +83,7 @@ int orc_poisson_table(double lambda, uint32_t *tab, int max_len);
 int orc_poisson_sample(const uint32_t *tab, uint32_t u);
 float orc_expf(float x);
 float orc_logf(float x);
+float orc_pow_ctrl(float e, float c);
 float orc_stdp_update(float w, float Dt, const float *param7);
 int orc_real_size(void);
 

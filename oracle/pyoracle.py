@@ -149,6 +149,8 @@ def lib(prec="f32"):
         L.orc_expf.argtypes = [f]
         L.orc_logf.restype = f
         L.orc_logf.argtypes = [f]
+        L.orc_pow_ctrl.restype = f
+        L.orc_pow_ctrl.argtypes = [f, f]
         L.orc_stdp_update.restype = f
         L.orc_stdp_update.argtypes = [f, f, vp]
         L.orc_beta_g0.restype = d

@@ -237,6 +237,24 @@ def test_logf_accuracy():
     assert L.orc_logf(1.0) == 0.0
 
 
+def test_pow_ctrl_accuracy():
+    """The step-size controller's err^-0.2 / err^-0.25 (orc_pow_ctrl, the
+    same sequence as dev_pow_ctrl) against math.pow, over the errors the
+    controller sees (above err_min = 1.89e-4) and well beyond."""
+    L = po.lib()
+    xs = np.concatenate([np.geomspace(1e-4, 1e8, 200001),
+                         np.linspace(0.5, 2.0, 20001)]).astype(np.float32)
+    for c in (-0.2, -0.25):
+        worst = 0.0
+        for x in xs:
+            ref = math.pow(float(x), c)
+            got = L.orc_pow_ctrl(float(x), c)
+            worst = max(worst, abs(got - ref) / ref)
+        assert worst <= 4e-7, (c, worst)
+        assert L.orc_pow_ctrl(float("inf"), c) == 0.0
+        assert math.isnan(L.orc_pow_ctrl(float("nan"), c))
+
+
 # ---------------------------------------------------------------------
 # STDP rule (src/stdp.cu:24-51) against the reference's test formula
 # ---------------------------------------------------------------------
